@@ -1,0 +1,134 @@
+/*
+ * ptt_hip.h -- the thin C-ABI between the C host layer (paratt.h API) and
+ * the hand-written gfx950 HIP kernels.  Plain pointers and sizes only.
+ *
+ * All kernel launchers are asynchronous on the library's single HIP stream
+ * (created at default priority on first use); ptt_hip_sync() waits for it.
+ * Column-major everywhere; "ld" is the leading dimension in elements.
+ * Any HIP runtime error is fatal: it is printed to stderr and the process
+ * exits with status 1 (the reference's error convention, paratt.c:1642-1645).
+ */
+#ifndef PARATT_AMD_PTT_HIP_H
+#define PARATT_AMD_PTT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime ---- */
+void ptt_hip_init(void);
+void *ptt_hip_malloc(size_t bytes);
+void ptt_hip_free(void *p);
+void ptt_hip_h2d(void *dst, const void *src, size_t bytes);       /* blocking */
+void ptt_hip_d2h(void *dst, const void *src, size_t bytes);       /* blocking */
+void ptt_hip_d2d(void *dst, const void *src, size_t bytes);       /* async */
+void ptt_hip_zero(void *p, size_t bytes);                         /* async fill kernel */
+void ptt_hip_sync(void);
+/* Grow-only device scratch slots owned by the library.  The returned pointer
+ * is valid until the next call for the same slot with a larger size. */
+enum { PTT_WS_GRAM = 0, PTT_WS_SMALL, PTT_WS_TMP, PTT_WS_TMP2, PTT_WS_INT,
+       PTT_WS_COEF, PTT_WS_SKETCH, PTT_WS_NSLOTS };
+void *ptt_hip_scratch(int slot, size_t bytes);
+/* Exchange the buffer of the TMP slot with a caller buffer: on return *buf
+ * points at the old scratch (capacity *cap_bytes updated) and the slot owns
+ * the caller's old buffer.  Used to make out-of-place results in place. */
+void ptt_hip_swap_tmp(int slot, double **buf, size_t *cap_bytes);
+size_t ptt_hip_scratch_cap(int slot);
+
+/* ---- profiling: HIP events around launches, per kernel name ---- */
+void ptt_hip_prof_enable(int on);
+void ptt_hip_prof_reset(void);
+/* Returns 1 and fills count / total_ms if `name` was recorded (syncs). */
+int ptt_hip_prof_get(const char *name, int *count, double *total_ms);
+/* Names recorded so far, '\n'-separated, into buf (truncated to len). */
+void ptt_hip_prof_names(char *buf, int len);
+/* Wall-clock of the stream between two markers, in ms (syncs). */
+void ptt_hip_timer_start(void);
+double ptt_hip_timer_stop(void);
+
+/* ---- streaming kernels (HBM-bound) ---- */
+/* Seeded N(0,1)*scale fill of `count` entries, element e of core `core`
+ * of the TT with seed `seed` (same stream as the CPU generator). */
+void ptt_hip_gauss_fill(double *dst, long count, uint64_t seed, int core,
+                        double scale);
+void ptt_hip_scale(double *p, long count, double alpha);
+/* Block placement of one core of z = a*x (+) b*y.
+ * kind 0: first core (column concat, scaled by a / b); kind 1: interior
+ * (block diagonal); kind 2: last core (row concat). */
+void ptt_hip_axpby_core(int kind, int n, int rx0, int rx1, const double *x,
+                        int ry0, int ry1, const double *y, double a, double b,
+                        double *z);
+/* Kronecker of every mode slice: z[(ry0*ax+ay), j, (ry1*bx+by)]. */
+void ptt_hip_hadamard_core(int n, int rx0, int rx1, const double *x, int ry0,
+                           int ry1, const double *y, double *z);
+/* f[a,j,b] *= v[j]  (v on device). */
+void ptt_hip_diag_core(int r0, int n, int r1, const double *v, double *f);
+/* Mode stencils, out-of-place: g = op(f).  coef on device.
+ * kind 0: pattac periodic (reference ghost-counter semantics),
+ * kind 1: attac periodic (exact wrap), kind 2: cdf (mirror ghosts). */
+void ptt_hip_stencil_core(int kind, int r0, int n, int r1, const double *sten,
+                          int slen, const double *f, double *g);
+/* Lax-Friedrichs-type flux (coef = [c(n), gc(2), s(n)]) and Lax-Wendroff
+ * RHS (coef = [c(n), gc(2), bw(n+1), s(n)]). */
+void ptt_hip_lf_flux_core(int r0, int n, int r1, const double *coef, double dt,
+                          double dx, const double *f, double *g);
+void ptt_hip_lw_rhs_core(int r0, int n, int r1, const double *coef, double dt,
+                         double dx, const double *f, double *g);
+/* Copy an m x n block with strides, optional scaling: B = alpha * A. */
+void ptt_hip_copy2d(int m, int n, const double *A, long lda, double *B, long ldb,
+                    double alpha);
+/* B (n x m) = alpha * A^T (A m x n). */
+void ptt_hip_transpose(int m, int n, const double *A, long lda, double *B,
+                       long ldb, double alpha);
+/* A[i + i*lda] = alpha for i < m (other entries untouched). */
+void ptt_hip_set_diag(double *A, int m, long lda, double alpha);
+/* Squared Frobenius norm of an m x n block, result to device *out. */
+void ptt_hip_sumsq(int m, int n, const double *A, long lda, double *out);
+
+/* ---- dense linear algebra (FP64, MFMA) ---- */
+/* C = alpha*op(A)*op(B) + beta*C.  ta/tb: 'N' or 'T'.  C may alias A (when
+ * N fits one column tile) or B (when M fits one row tile): see gemm.hip. */
+void ptt_hip_dgemm(char ta, char tb, int M, int N, int K, double alpha,
+                   const double *A, long lda, const double *B, long ldb,
+                   double beta, double *C, long ldc);
+/* Gram matrices with a split over the long dimension:
+ * trans 'T': G (c x c) = A^T A for A (m x c, lda);
+ * trans 'N': G (c x c) = A A^T for A (c x w, lda). */
+void ptt_hip_gram(char trans, int c, long len, const double *A, long lda,
+                  double *G);
+
+/* Jacobi-scaled, guarded Cholesky of a c x c SPD Gram G (ld c):
+ * D = sqrt(diag G); (D^-1 G D^-1 + shift*I) = Rs^T Rs; pivots below
+ * `tiny` mark the index deficient (row of R and column of T set to 0).
+ * Outputs (ld c): R = Rs*D (upper), T = D^-1 * Rs^-1 (upper, deficient
+ * columns 0) so that A*T has orthonormal columns and A = (A*T)*R.
+ * info[0] = number of deficient pivots, dinfo[0] = ||Rs||_F*||Rs^-1||_F. */
+void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny,
+                            double *R, double *T, int *info, double *dinfo);
+
+/* One-sided Jacobi SVD of A (m x k, lda, m >= k), FP64.
+ * On return: sig[0..k) descending, U (m x k, ldu) left vectors,
+ * V (k x k, ldv) right vectors if V != NULL.  Also the numerical rank
+ * numrank (sig_j > sqrt(k)*eps*sig_0) and the truncation rank by the
+ * reference's tail rule (pattac.c:468-481) applied to sig*exp(log_scale)
+ * against delta, capped by min(k, cap), floor 1: written to iout[0]=numrank,
+ * iout[1]=rho, iout[2]=sweeps.  A is overwritten. */
+void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig,
+                        double *U, long ldu, double *V, long ldv,
+                        double log_scale, double delta, int cap, int *iout);
+/* From U (m x k, ldu) and sig: W (rho x m, ld rho) = diag(1/sig) U_rho^T
+ * scaled by wscale, and US (m x rho, ld m) = U_rho diag(sig).
+ * rho is read from device *drho.  Either output may be NULL. */
+void ptt_hip_svd_factors(int m, int rhomax, const int *drho, const double *U,
+                         long ldu, const double *sig, double wscale, double *W,
+                         double *US);
+/* Small c x c triangular product C = A*B (upper times upper), ld c. */
+void ptt_hip_trmm_small(int c, const double *A, const double *B, double *C);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,244 @@
+"""paratt_amd: MI355X-native tensor-train hot path (TT rounding, formal
+axpby / Hadamard, mode stencils, ODE steps) behind the paratt.h C API.
+
+This module is a thin ctypes binding of paratt_amd/libparatt_amd.so (built
+from paratt_amd/csrc by `make -C paratt_amd`).  It mirrors the reference's
+paratt.h operator interface (same names, argument meaning, error behaviour:
+the C library prints and exits on fatal errors) for Python callers and the
+tests.  There is no CPU fallback: if the shared library is missing, importing
+the binding raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libparatt_amd.so")
+
+# Symbols every build must export: the paratt.h / paratt_ode.h / pattac.h
+# entry points of the hot path plus the C-ABI of include/ptt_hip.h.
+PUBLIC_SYMBOLS = [
+    "alloc_tt", "alloc_ptt", "rank_change_tt", "copy_rank_change_tt", "free_tt", "free_ptt",
+    "attac_tt_axpby", "attac_tt_scale", "paratt_truncate", "paratt_norm", "paratt_dotprod",
+    "attac_set_rank1", "attac_sum_of_product", "attac_mult_diag_mat",
+    "attac_apply_periodic_stencil", "pattac_apply_periodic_stencil", "pattac_apply_cdf_stencil",
+    "pattac_apply_cdf_lf_flux", "pattac_apply_cdf_lw_rhs",
+    "pattac_exp_euler_st", "attac_exp_euler_st",
+    "attacFormalAXPBY", "pattacFormalAXPBY", "attacFormalSum", "pattacFormalSum",
+    "attacHadamard", "pattacHadamard", "pattacLOrthogonalization", "pattacLTruncate",
+    "pattacTruncate", "pattacInnerProduct",
+    "paratt_amd_core_upload", "paratt_amd_core_download", "paratt_amd_fill_gaussian",
+    "paratt_amd_truncate_rel", "paratt_amd_last_round_stats", "paratt_amd_sync",
+]
+
+
+class AttacTT(C.Structure):
+    """Byte layout of attac_tt (source/paratt.h:25-51)."""
+    _fields_ = [("numDim", C.c_int), ("dim", C.POINTER(C.c_int)),
+                ("ttRank", C.POINTER(C.c_int)), ("coreList", C.POINTER(C.c_void_p)),
+                ("coreMem", C.POINTER(C.c_int)), ("wk", C.POINTER(C.c_double)),
+                ("wksz", C.c_int)]
+
+
+class Paratt(C.Structure):
+    """Byte layout of paratt (source/paratt.h:53-63)."""
+    _fields_ = [("x", C.POINTER(AttacTT)), ("mpi_rk", C.c_int), ("mpi_sz", C.c_int),
+                ("distrib_dim_mat", C.POINTER(C.c_int)), ("dim", C.POINTER(C.c_int)),
+                ("numDim", C.c_int), ("iwk", C.POINTER(C.c_int)), ("iwksz", C.c_int),
+                ("orthog_wk", C.c_void_p)]
+
+
+class RoundStats(C.Structure):
+    _fields_ = [("d", C.c_int), ("ltr_cholqr_passes", C.c_int * 64),
+                ("rtl_cholqr_passes", C.c_int * 64), ("svd_sweeps", C.c_int * 64),
+                ("numrank", C.c_int * 64), ("norm", C.c_double),
+                ("sig", (C.c_double * 8) * 64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("paratt_amd: %s not built (run `make -C paratt_amd`)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER(Paratt)
+        T = C.POINTER(AttacTT)
+        IP = C.POINTER(C.c_int)
+        DP = C.POINTER(C.c_double)
+        L.alloc_ptt.restype = P
+        L.alloc_ptt.argtypes = [C.c_int, IP, IP]
+        L.alloc_tt.restype = T
+        L.alloc_tt.argtypes = [C.c_int, IP, IP]
+        L.free_ptt.argtypes = [P]
+        L.free_tt.argtypes = [T]
+        L.rank_change_tt.restype = T
+        L.rank_change_tt.argtypes = [T, IP]
+        L.copy_rank_change_tt.argtypes = [T, T]
+        L.attac_tt_axpby.argtypes = [C.c_double, T, C.c_double, T, T]
+        L.attac_tt_scale.argtypes = [C.c_double, T]
+        L.paratt_truncate.argtypes = [P, C.c_double, IP]
+        L.paratt_amd_truncate_rel.argtypes = [P, C.c_double, IP]
+        L.paratt_norm.restype = C.c_double
+        L.paratt_norm.argtypes = [P]
+        L.paratt_dotprod.restype = C.c_double
+        L.paratt_dotprod.argtypes = [P, P]
+        L.attac_mult_diag_mat.argtypes = [T, C.c_int, DP]
+        L.attac_apply_periodic_stencil.argtypes = [T, C.c_int, DP, C.c_int]
+        L.pattac_apply_periodic_stencil.argtypes = [P, C.c_int, DP, C.c_int]
+        L.pattac_apply_cdf_stencil.argtypes = [P, C.c_int, DP, C.c_int]
+        L.pattac_apply_cdf_lf_flux.argtypes = [P, C.c_int, DP, DP, DP, C.c_double, C.c_double]
+        L.pattac_apply_cdf_lw_rhs.argtypes = [P, C.c_int, DP, DP, DP, DP, C.c_double,
+                                              C.c_double]
+        L.pattacHadamard.argtypes = [C.c_int, IP, IP, C.POINTER(C.c_void_p), IP,
+                                     C.POINTER(C.c_void_p), IP, C.POINTER(C.c_void_p)]
+        L.pattacInnerProduct.restype = C.c_double
+        L.paratt_amd_core_upload.argtypes = [T, C.c_int, DP]
+        L.paratt_amd_core_download.argtypes = [T, C.c_int, DP]
+        L.paratt_amd_fill_gaussian.argtypes = [T, C.c_uint64, C.c_double]
+        L.paratt_amd_last_round_stats.argtypes = [C.POINTER(RoundStats)]
+        L.ptt_hip_prof_enable.argtypes = [C.c_int]
+        L.ptt_hip_prof_get.restype = C.c_int
+        L.ptt_hip_prof_get.argtypes = [C.c_char_p, IP, DP]
+        L.ptt_hip_prof_names.argtypes = [C.c_char_p, C.c_int]
+        L.ptt_hip_timer_stop.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _ia(v):
+    return (C.c_int * len(v))(*[int(t) for t in v])
+
+
+def _dp(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class DevTT:
+    """A GPU-resident TT (`paratt*` from alloc_ptt)."""
+
+    def __init__(self, n, r):
+        self.p = lib().alloc_ptt(len(n), _ia(n), _ia(r))
+
+    @property
+    def d(self):
+        return self.p.contents.numDim
+
+    @property
+    def n(self):
+        x = self.p.contents.x.contents
+        return [x.dim[k] for k in range(x.numDim)]
+
+    @property
+    def r(self):
+        x = self.p.contents.x.contents
+        return [x.ttRank[k] for k in range(x.numDim + 1)]
+
+    @property
+    def tt(self):
+        return self.p.contents.x
+
+    def core_size(self, k):
+        r = self.r
+        return r[k] * self.n[k] * r[k + 1]
+
+    def upload(self, host):
+        """Copy a host TT (object with .n, .r, .cores) into this TT, resizing."""
+        assert list(host.n) == self.n
+        lib().rank_change_tt(self.tt, _ia(host.r))
+        for k in range(self.d):
+            c = np.ascontiguousarray(host.cores[k][: host.r[k] * host.n[k] * host.r[k + 1]],
+                                     dtype=np.float64)
+            lib().paratt_amd_core_upload(self.tt, k, c.ctypes.data_as(C.POINTER(C.c_double)))
+        return self
+
+    def download_cores(self):
+        out = []
+        for k in range(self.d):
+            c = np.empty(max(self.core_size(k), 1))
+            lib().paratt_amd_core_download(self.tt, k, c.ctypes.data_as(C.POINTER(C.c_double)))
+            out.append(c[: self.core_size(k)])
+        return out
+
+    def free(self):
+        if self.p:
+            lib().free_ptt(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @staticmethod
+    def from_host(host):
+        return DevTT(host.n, host.r).upload(host)
+
+    @staticmethod
+    def gaussian(n, r, seed, scale=1.0):
+        t = DevTT(n, r)
+        lib().paratt_amd_fill_gaussian(t.tt, C.c_uint64(seed), C.c_double(scale))
+        return t
+
+
+def axpby(a, x, b, y):
+    d = x.d
+    rz = [1] + [x.r[k] + y.r[k] for k in range(1, d)] + [1]
+    z = DevTT(x.n, rz)
+    lib().attac_tt_axpby(a, x.tt, b, y.tt, z.tt)
+    return z
+
+
+def truncate(x, tol, maxr=None):
+    mr = _ia(maxr if maxr is not None else x.r)
+    lib().paratt_truncate(x.p, tol, mr)
+
+
+def truncate_rel(x, eps, maxr=None):
+    mr = _ia(maxr if maxr is not None else x.r)
+    lib().paratt_amd_truncate_rel(x.p, eps, mr)
+
+
+def norm(x):
+    return lib().paratt_norm(x.p)
+
+
+def dotprod(x, y):
+    return lib().paratt_dotprod(x.p, y.p)
+
+
+def last_round_stats():
+    s = RoundStats()
+    lib().paratt_amd_last_round_stats(C.byref(s))
+    return s
+
+
+def sync():
+    lib().paratt_amd_sync()
+
+
+def prof_enable(on=True):
+    lib().ptt_hip_prof_enable(1 if on else 0)
+
+
+def prof_reset():
+    lib().ptt_hip_prof_reset()
+
+
+def prof_get(name):
+    cnt = C.c_int(0)
+    ms = C.c_double(0.0)
+    lib().ptt_hip_prof_get(name.encode(), C.byref(cnt), C.byref(ms))
+    return cnt.value, ms.value
+
+
+def prof_names():
+    buf = C.create_string_buffer(8192)
+    lib().ptt_hip_prof_names(buf, 8192)
+    return [s for s in buf.value.decode().split("\n") if s]

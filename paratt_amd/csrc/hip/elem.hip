@@ -1,0 +1,391 @@
+// elem.hip -- streaming (HBM-bound) kernels on TT cores for gfx950.
+//
+// Core k is column-major r0 x n x r1; element (a, j, b) at a + r0*(j + n*b).
+// Every kernel is output-stationary with the fastest-varying rank index a
+// mapped to consecutive lanes, so loads and stores coalesce.  The stencil,
+// flux and diagonal kernels compile with FP contraction off so each output
+// is rounded exactly as the reference's scalar C loops round it.
+#include "common.hpp"
+
+using namespace ptt;
+
+namespace {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_gauss_fill(double *dst, long count, uint64_t h, double scale) {
+    long npair = (count + 1) / 2;
+    for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < npair;
+         p += (long)gridDim.x * blockDim.x) {
+        uint64_t a = mix64(h + 2 * (uint64_t)p), b = mix64(h + 2 * (uint64_t)p + 1);
+        double u1 = ((double)(a >> 11) + 0.5) * 0x1.0p-53;
+        double u2 = (double)(b >> 11) * 0x1.0p-53;
+        double rad = sqrt(-2.0 * log(u1));
+        double th = 6.283185307179586476925286766559 * u2;
+        double s, c;
+        sincos(th, &s, &c);
+        dst[2 * p] = rad * c * scale;
+        if (2 * p + 1 < count) dst[2 * p + 1] = rad * s * scale;
+    }
+}
+
+__global__ void k_scale(double *p, long count, double alpha) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < count;
+         e += (long)gridDim.x * blockDim.x)
+        p[e] *= alpha;
+}
+
+__global__ void k_zero(double *p, long count) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < count;
+         e += (long)gridDim.x * blockDim.x)
+        p[e] = 0.0;
+}
+
+// z = a*x (+) b*y, one core.  kind 0 first (cols concat), 1 interior
+// (block diagonal), 2 last (rows concat).  attac.c:81-112.
+__global__ void k_axpby_core(int kind, int n, int rx0, int rx1, const double *__restrict__ x,
+                             int ry0, int ry1, const double *__restrict__ y, double a,
+                             double b, double *__restrict__ z) {
+    int rz0 = (kind == 0) ? 1 : rx0 + ry0;
+    int rz1 = (kind == 2) ? 1 : rx1 + ry1;
+    long rows = (long)rz0 * n;
+    long total = rows * rz1;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        long col = e / rows;
+        long m = e - col * rows;
+        int j = (int)(m / rz0);
+        int al = (int)(m - (long)j * rz0);
+        double v = 0.0;
+        if (kind == 0) {
+            v = (col < rx1) ? x[j + (long)n * col] * a
+                            : y[j + (long)n * (col - rx1)] * b;
+        } else if (kind == 2) {
+            v = (al < rx0) ? x[al + (long)rx0 * j] : y[(al - rx0) + (long)ry0 * j];
+        } else {
+            if (al < rx0 && col < rx1)
+                v = x[al + (long)rx0 * (j + (long)n * col)];
+            else if (al >= rx0 && col >= rx1)
+                v = y[(al - rx0) + (long)ry0 * (j + (long)n * (col - rx1))];
+        }
+        z[e] = v;
+    }
+}
+
+// z[(ry0*ax+ay), j, (ry1*bx+by)] = x[ax,j,bx] * y[ay,j,by] (utilities.c:112-129)
+__global__ void k_hadamard_core(int n, int rx0, int rx1, const double *__restrict__ x,
+                                int ry0, int ry1, const double *__restrict__ y,
+                                double *__restrict__ z) {
+    int rz0 = rx0 * ry0;
+    long rows = (long)rz0 * n;
+    long total = rows * ((long)rx1 * ry1);
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        long col = e / rows;
+        long m = e - col * rows;
+        int j = (int)(m / rz0);
+        int al = (int)(m - (long)j * rz0);
+        int ax = al / ry0, ay = al - ax * ry0;
+        int bx = (int)(col / ry1), by = (int)(col - (long)bx * ry1);
+        z[e] = x[ax + (long)rx0 * (j + (long)n * bx)] * y[ay + (long)ry0 * (j + (long)n * by)];
+    }
+}
+
+#pragma clang fp contract(off)
+
+__global__ void k_diag_core(int r0, int n, int r1, const double *__restrict__ v,
+                            double *__restrict__ f) {
+    long rows = (long)r0 * n, total = rows * r1;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        long m = e % rows;
+        int j = (int)(m / r0);
+        f[e] *= v[j];
+    }
+}
+
+// Mode stencils: paratt.c:584-661 (kind 1), :665-770 (kind 0), :772-923 (kind 2).
+__global__ void k_stencil_core(int kind, int r0, int n, int r1,
+                               const double *__restrict__ sten, int slen,
+                               const double *__restrict__ f, double *__restrict__ g) {
+    const int h = (slen - 1) / 2;
+    long rows = (long)r0 * n, total = rows * r1;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        long col = e / rows;
+        long m = e - col * rows;
+        int i = (int)(m / r0);
+        int al = (int)(m - (long)i * r0);
+        const double *fb = f + al + col * rows;  // fiber base; stride r0
+#define FV(jj) fb[(long)r0 * (jj)]
+        double acc = 0.0;
+        if (kind == 1) {
+            // exact periodic wrap, summed in stencil order
+            for (int t = 0; t < slen; t++) {
+                int jj = i + t - h;
+                jj = jj < 0 ? jj + n : (jj >= n ? jj - n : jj);
+                acc += sten[t] * FV(jj);
+            }
+        } else {
+            int sc = 0, gc = 0;
+            for (int jj = i - h; jj < i; jj++) {
+                double v;
+                if (jj < 0) {
+                    v = (kind == 0) ? FV(n - h + gc) : FV(h - 1 - gc);
+                    gc++;
+                } else {
+                    v = FV(jj);
+                }
+                acc += sten[sc++] * v;
+            }
+            acc += sten[sc++] * FV(i);
+            gc = 0;
+            for (int jj = i + 1; jj <= i + h; jj++) {
+                double v;
+                if (jj < n) {
+                    v = FV(jj);
+                } else {
+                    v = (kind == 0) ? FV(gc) : FV(n - 1 - gc);
+                    gc++;
+                }
+                acc += sten[sc++] * v;
+            }
+        }
+        g[e] = acc;
+#undef FV
+    }
+}
+
+// paratt.c:926-1093 at one rank; coef = [c(n), gc(2), s(n)].
+__global__ void k_lf_flux_core(int r0, int n, int r1, const double *__restrict__ coef,
+                               double dx, const double *__restrict__ f,
+                               double *__restrict__ g) {
+    const double *c = coef, *gcw = coef + n, *s = coef + n + 2;
+    const double s0 = -0.5 / dx, s2 = 0.5 / dx;
+    long rows = (long)r0 * n, total = rows * r1;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        long col = e / rows;
+        long m = e - col * rows;
+        int i = (int)(m / r0);
+        int al = (int)(m - (long)i * r0);
+        const double *fb = f + al + col * rows;
+#define FV(jj) fb[(long)r0 * (jj)]
+        double v = (i - 1 < 0) ? -gcw[0] * FV(0) : -c[i - 1] * FV(i - 1);
+        double acc = 0.0;
+        acc += s0 * v;
+        v = (i + 1 < n) ? -c[i + 1] * FV(i + 1) : -gcw[1] * FV(n - 1);
+        acc += s2 * v;
+        g[e] = acc + s[i] * FV(i);
+#undef FV
+    }
+}
+
+// paratt.c:1096-1250 at one rank; coef = [c(n), gc(2), bw(n+1), s(n)].
+__global__ void k_lw_rhs_core(int r0, int n, int r1, const double *__restrict__ coef,
+                              double dt, double dx, const double *__restrict__ f,
+                              double *__restrict__ g) {
+    const double *c = coef, *gcw = coef + n, *bw = coef + n + 2, *s = coef + 2 * n + 3;
+    long rows = (long)r0 * n, total = rows * r1;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        long col = e / rows;
+        long m = e - col * rows;
+        int i = (int)(m / r0);
+        int al = (int)(m - (long)i * r0);
+        const double *fb = f + al + col * rows;
+#define FV(jj) fb[(long)r0 * (jj)]
+        double uh0, uh1;
+        if (i == 0) {
+            double g0 = FV(0);
+            uh0 = 0.5 * (g0 + FV(0)) - ((c[0] * FV(0) - g0 * gcw[0]) * dt / (2.0 * dx));
+        } else {
+            uh0 = 0.5 * (FV(i) + FV(i - 1)) -
+                  ((c[i] * FV(i) - c[i - 1] * FV(i - 1)) * dt / (2.0 * dx));
+        }
+        if (i + 1 < n) {
+            uh1 = 0.5 * (FV(i + 1) + FV(i)) -
+                  ((c[i + 1] * FV(i + 1) - c[i] * FV(i)) * dt / (2.0 * dx));
+        } else {
+            double g1 = FV(n - 1);
+            uh1 = 0.5 * (g1 + FV(i)) - ((g1 * gcw[1] - c[i] * FV(i)) * dt / (2.0 * dx));
+        }
+        g[e] = -(bw[i + 1] * uh1 - bw[i] * uh0) / dx + s[i] * FV(i);
+#undef FV
+    }
+}
+
+#pragma clang fp contract(on)
+
+__global__ void k_copy2d(int m, int n, const double *__restrict__ A, long lda,
+                         double *__restrict__ B, long ldb, double alpha) {
+    long total = (long)m * n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        long j = e / m, i = e - j * m;
+        B[i + j * ldb] = alpha * A[i + j * lda];
+    }
+}
+
+// B (n x m) = alpha * A^T, A is m x n; 32x32 tiles through LDS.
+__global__ void k_transpose(int m, int n, const double *__restrict__ A, long lda,
+                            double *__restrict__ B, long ldb, double alpha) {
+    __shared__ double tile[32][33];
+    int bi = blockIdx.x * 32, bj = blockIdx.y * 32;
+    int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: ty 0..7
+    for (int yy = ty; yy < 32; yy += 8) {
+        int i = bi + tx, j = bj + yy;
+        tile[yy][tx] = (i < m && j < n) ? A[i + (long)j * lda] : 0.0;
+    }
+    __syncthreads();
+    for (int yy = ty; yy < 32; yy += 8) {
+        int j = bj + tx, i = bi + yy;  // B(j, i) = A(i, j)
+        if (j < n && i < m) B[j + (long)i * ldb] = alpha * tile[tx][yy];
+    }
+}
+
+__global__ void k_set_diag(double *A, int m, long lda, double alpha) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+        A[i + (long)i * lda] = alpha;
+}
+
+__global__ void k_sumsq(int m, int n, const double *__restrict__ A, long lda,
+                        double *out) {
+    __shared__ double red[1024 / 64];
+    double s = 0.0;
+    long total = (long)m * n;
+    for (long e = threadIdx.x; e < total; e += blockDim.x) {
+        long j = e / m, i = e - j * m;
+        double v = A[i + j * lda];
+        s += v * v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += red[i];
+        *out = t;
+    }
+}
+
+uint64_t mix64_host(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+uint64_t mix64_host_seed(uint64_t seed, int core) {
+    return mix64_host(mix64_host(seed) + (uint64_t)core);
+}
+
+}  // namespace
+
+extern "C" {
+
+void ptt_hip_zero(void *p, size_t bytes) {
+    ptt_hip_init();
+    long cnt = (long)(bytes / sizeof(double));
+    if (cnt <= 0) return;
+    PTT_LAUNCH("zero", k_zero, dim3(stream_blocks(cnt, 256)), dim3(256), 0, (double *)p, cnt);
+}
+
+void ptt_hip_gauss_fill(double *dst, long count, uint64_t seed, int core, double scale) {
+    ptt_hip_init();
+    if (count <= 0) return;
+    uint64_t h = mix64_host_seed(seed, core);
+    long np = (count + 1) / 2;
+    PTT_LAUNCH("gauss_fill", k_gauss_fill, dim3(stream_blocks(np, 256)), dim3(256), 0, dst,
+               count, h, scale);
+}
+
+void ptt_hip_scale(double *p, long count, double alpha) {
+    ptt_hip_init();
+    if (count <= 0) return;
+    PTT_LAUNCH("scale", k_scale, dim3(stream_blocks(count, 256)), dim3(256), 0, p, count, alpha);
+}
+
+void ptt_hip_axpby_core(int kind, int n, int rx0, int rx1, const double *x, int ry0, int ry1,
+                        const double *y, double a, double b, double *z) {
+    ptt_hip_init();
+    long rz0 = (kind == 0) ? 1 : rx0 + ry0, rz1 = (kind == 2) ? 1 : rx1 + ry1;
+    long total = rz0 * n * rz1;
+    PTT_LAUNCH("axpby", k_axpby_core, dim3(stream_blocks(total, 256)), dim3(256), 0, kind, n,
+               rx0, rx1, x, ry0, ry1, y, a, b, z);
+}
+
+void ptt_hip_hadamard_core(int n, int rx0, int rx1, const double *x, int ry0, int ry1,
+                           const double *y, double *z) {
+    ptt_hip_init();
+    long total = (long)rx0 * ry0 * n * rx1 * ry1;
+    PTT_LAUNCH("hadamard", k_hadamard_core, dim3(stream_blocks(total, 256)), dim3(256), 0, n,
+               rx0, rx1, x, ry0, ry1, y, z);
+}
+
+void ptt_hip_diag_core(int r0, int n, int r1, const double *v, double *f) {
+    ptt_hip_init();
+    long total = (long)r0 * n * r1;
+    PTT_LAUNCH("diag", k_diag_core, dim3(stream_blocks(total, 256)), dim3(256), 0, r0, n, r1, v,
+               f);
+}
+
+void ptt_hip_stencil_core(int kind, int r0, int n, int r1, const double *sten, int slen,
+                          const double *f, double *g) {
+    ptt_hip_init();
+    long total = (long)r0 * n * r1;
+    PTT_LAUNCH("stencil", k_stencil_core, dim3(stream_blocks(total, 256)), dim3(256), 0, kind,
+               r0, n, r1, sten, slen, f, g);
+}
+
+void ptt_hip_lf_flux_core(int r0, int n, int r1, const double *coef, double dt, double dx,
+                          const double *f, double *g) {
+    ptt_hip_init();
+    (void)dt;
+    long total = (long)r0 * n * r1;
+    PTT_LAUNCH("lf_flux", k_lf_flux_core, dim3(stream_blocks(total, 256)), dim3(256), 0, r0, n,
+               r1, coef, dx, f, g);
+}
+
+void ptt_hip_lw_rhs_core(int r0, int n, int r1, const double *coef, double dt, double dx,
+                         const double *f, double *g) {
+    ptt_hip_init();
+    long total = (long)r0 * n * r1;
+    PTT_LAUNCH("lw_rhs", k_lw_rhs_core, dim3(stream_blocks(total, 256)), dim3(256), 0, r0, n,
+               r1, coef, dt, dx, f, g);
+}
+
+void ptt_hip_copy2d(int m, int n, const double *A, long lda, double *B, long ldb,
+                    double alpha) {
+    ptt_hip_init();
+    long total = (long)m * n;
+    if (total <= 0) return;
+    PTT_LAUNCH("copy2d", k_copy2d, dim3(stream_blocks(total, 256)), dim3(256), 0, m, n, A, lda,
+               B, ldb, alpha);
+}
+
+void ptt_hip_transpose(int m, int n, const double *A, long lda, double *B, long ldb,
+                       double alpha) {
+    ptt_hip_init();
+    if (m <= 0 || n <= 0) return;
+    PTT_LAUNCH("transpose", k_transpose, dim3(cdiv(m, 32), cdiv(n, 32)), dim3(256), 0, m, n, A,
+               lda, B, ldb, alpha);
+}
+
+void ptt_hip_set_diag(double *A, int m, long lda, double alpha) {
+    ptt_hip_init();
+    if (m <= 0) return;
+    PTT_LAUNCH("set_diag", k_set_diag, dim3(cdiv(m, 256)), dim3(256), 0, A, m, lda, alpha);
+}
+
+void ptt_hip_sumsq(int m, int n, const double *A, long lda, double *out) {
+    ptt_hip_init();
+    PTT_LAUNCH("sumsq", k_sumsq, dim3(1), dim3(1024), 0, m, n, A, lda, out);
+}
+
+}  // extern "C"

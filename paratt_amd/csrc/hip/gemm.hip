@@ -1,0 +1,255 @@
+// gemm.hip -- FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4f64).
+//
+// C = alpha*op(A)*op(B) + beta*C, column-major.  A workgroup of 4 waves owns
+// a BM x BN tile of C; each wave owns (BM/2) x (BN/2) as 16x16 MFMA blocks.
+// The MFMA is issued as D^T = op(B)^T op(A)^T so that the 16 lanes of a
+// row group hold 16 consecutive m (rows of column-major C): every store of
+// the epilogue writes four full 128-byte column segments.
+// K advances in steps of 16 through padded LDS tiles, with the next step's
+// global loads held in registers while the current step computes.
+// Split-K (gridDim.z > 1) writes partial tiles to a scratch buffer that a
+// second kernel reduces; SYRK mode skips waves strictly below the diagonal
+// and the reduction mirrors the upper triangle.
+#include "common.hpp"
+
+using namespace ptt;
+
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BK = 16;
+constexpr int PAD = 2;
+
+template <int BM, int BN, bool TA, bool TB, bool SYRK>
+__global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, int kchunk,
+                                                double alpha, const double *__restrict__ A,
+                                                long lda, const double *__restrict__ B,
+                                                long ldb, double beta, double *C, long ldc,
+                                                double *__restrict__ partial) {
+    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int TM = WM / 16, TN = WN / 16;
+    constexpr int AE = BM * BK / 256, BE = BN * BK / 256;
+    __shared__ double As[BK][BM + PAD];
+    __shared__ double Bs[BK][BN + PAD];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+    // SYRK: a wave whose whole m-range lies below its n-range computes nothing.
+    const bool active = !(SYRK && (m0 + wm * WM) > (n0 + wn * WN + WN - 1));
+
+    f64x4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+    double ra[AE], rb[BE];
+
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < AE; e++) {
+            int idx = tid + 256 * e;
+            int mm, kk;
+            if (TA) { kk = idx % BK; mm = idx / BK; }
+            else    { mm = idx % BM; kk = idx / BM; }
+            int gm = m0 + mm, gk = k0 + kk;
+            double v = 0.0;
+            if (gm < M && gk < kend)
+                v = TA ? A[gk + (long)gm * lda] : A[gm + (long)gk * lda];
+            ra[e] = v;
+        }
+#pragma unroll
+        for (int e = 0; e < BE; e++) {
+            int idx = tid + 256 * e;
+            int nn, kk;
+            if (TB) { nn = idx % BN; kk = idx / BN; }
+            else    { kk = idx % BK; nn = idx / BK; }
+            int gn = n0 + nn, gk = k0 + kk;
+            double v = 0.0;
+            if (gn < N && gk < kend)
+                v = TB ? B[gn + (long)gk * ldb] : B[gk + (long)gn * ldb];
+            rb[e] = v;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int e = 0; e < AE; e++) {
+            int idx = tid + 256 * e;
+            int mm, kk;
+            if (TA) { kk = idx % BK; mm = idx / BK; }
+            else    { mm = idx % BM; kk = idx / BM; }
+            As[kk][mm] = ra[e];
+        }
+#pragma unroll
+        for (int e = 0; e < BE; e++) {
+            int idx = tid + 256 * e;
+            int nn, kk;
+            if (TB) { nn = idx % BN; kk = idx / BN; }
+            else    { kk = idx % BK; nn = idx / BK; }
+            Bs[kk][nn] = rb[e];
+        }
+    };
+
+    if (kbeg < kend) load(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        __syncthreads();
+        store();
+        __syncthreads();
+        if (k0 + BK < kend) load(k0 + BK);
+        if (active) {
+#pragma unroll
+            for (int ks = 0; ks < BK / 4; ks++) {
+                const int kr = ks * 4 + (lane >> 4);
+                double af[TM], bf[TN];
+#pragma unroll
+                for (int i = 0; i < TM; i++) af[i] = As[kr][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < TN; j++) bf[j] = Bs[kr][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i],
+                                                                         acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+    if (!active) return;
+
+    // acc[i][j][v] = C[m = m0+wm*WM+i*16+(lane&15)][n = n0+wn*WN+j*16+4*(lane>>4)+v]
+#pragma unroll
+    for (int i = 0; i < TM; i++) {
+        const int gm = m0 + wm * WM + i * 16 + (lane & 15);
+        if (gm >= M) continue;
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int gn = n0 + wn * WN + j * 16 + 4 * (lane >> 4) + v;
+                if (gn >= N) continue;
+                double r = acc[i][j][v];
+                if (partial) {
+                    partial[(long)blockIdx.z * M * N + gm + (long)gn * M] = r;
+                } else {
+                    double* cp = C + gm + (long)gn * ldc;
+                    *cp = (beta == 0.0) ? alpha * r : alpha * r + beta * (*cp);
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_splitk_reduce(int M, int N, int nsplit, const double *__restrict__ partial,
+                                double alpha, double beta, double *C, long ldc, int sym) {
+    long total = (long)M * N;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        int n = (int)(e / M), m = (int)(e - (long)n * M);
+        long src = (sym && m > n) ? (long)n + (long)m * M : e;
+        double s = 0.0;
+        for (int z = 0; z < nsplit; z++) s += partial[(long)z * total + src];
+        double *cp = C + m + (long)n * ldc;
+        *cp = (beta == 0.0) ? alpha * s : alpha * s + beta * (*cp);
+    }
+}
+
+template <int BM, int BN, bool TA, bool TB, bool SYRK>
+void launch_t(dim3 grid, int M, int N, int K, int kchunk, double alpha, const double *A,
+              long lda, const double *B, long ldb, double beta, double *C, long ldc,
+              double *partial, const char *name) {
+    PTT_LAUNCH(name, (k_dgemm<BM, BN, TA, TB, SYRK>), grid, dim3(256), 0, M, N, K, kchunk,
+               alpha, A, lda, B, ldb, beta, C, ldc, partial);
+}
+
+template <int BM, int BN, bool SYRK>
+void launch_bt(char ta, char tb, dim3 grid, int M, int N, int K, int kchunk, double alpha,
+               const double *A, long lda, const double *B, long ldb, double beta, double *C,
+               long ldc, double *partial, const char *name) {
+    bool TA = (ta == 'T' || ta == 't'), TB = (tb == 'T' || tb == 't');
+    if (!TA && !TB)
+        launch_t<BM, BN, false, false, SYRK>(grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name);
+    else if (TA && !TB)
+        launch_t<BM, BN, true, false, SYRK>(grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name);
+    else if (!TA && TB)
+        launch_t<BM, BN, false, true, SYRK>(grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name);
+    else
+        launch_t<BM, BN, true, true, SYRK>(grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name);
+}
+
+void gemm_impl(char ta, char tb, int M, int N, int K, double alpha, const double *A, long lda,
+               const double *B, long ldb, double beta, double *C, long ldc, bool syrk,
+               bool force_split, const char *name) {
+    ptt_hip_init();
+    if (M <= 0 || N <= 0) return;
+    if (K <= 0) {
+        // C = beta*C
+        if (beta == 0.0) {
+            for (int n = 0; n < N; n++) ptt_hip_zero(C + (long)n * ldc, sizeof(double) * M);
+        } else if (beta != 1.0) {
+            ptt_hip_copy2d(M, N, C, ldc, C, ldc, beta);
+        }
+        return;
+    }
+    const int BM = (M <= 64) ? 64 : 128, BN = (N <= 64) ? 64 : 128;
+    const unsigned gm = cdiv(M, BM), gn = cdiv(N, BN);
+    int nsplit = 1;
+    long tiles = (long)gm * gn;
+    if ((force_split || tiles < 64) && K >= 1024) {
+        // aim at ~2 waves of workgroups over the 256 CUs, >= 256 of K each
+        long want = (512 + tiles - 1) / tiles;
+        long maxs = K / 256;
+        nsplit = (int)(want < maxs ? want : maxs);
+        if (nsplit < 1) nsplit = 1;
+        if (nsplit > 1024) nsplit = 1024;
+    }
+    if (nsplit == 1) syrk = false;  // the mirror lives in the split-K reduction
+    int kchunk = K;
+    double *partial = nullptr;
+    if (nsplit > 1) {
+        kchunk = (int)(((long)K + nsplit - 1) / nsplit);
+        kchunk = (kchunk + BK - 1) / BK * BK;
+        nsplit = (int)(((long)K + kchunk - 1) / kchunk);
+        partial = (double *)ptt_hip_scratch(PTT_WS_GRAM, sizeof(double) * (size_t)nsplit * M * N);
+    }
+    dim3 grid(gm, gn, nsplit);
+    if (BM == 64 && BN == 64)
+        syrk ? launch_bt<64, 64, true>(ta, tb, grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name)
+             : launch_bt<64, 64, false>(ta, tb, grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name);
+    else if (BM == 64)
+        launch_bt<64, 128, false>(ta, tb, grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name);
+    else if (BN == 64)
+        launch_bt<128, 64, false>(ta, tb, grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name);
+    else
+        syrk ? launch_bt<128, 128, true>(ta, tb, grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name)
+             : launch_bt<128, 128, false>(ta, tb, grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name);
+    if (nsplit > 1) {
+        long total = (long)M * N;
+        PTT_LAUNCH("splitk_reduce", k_splitk_reduce, dim3(stream_blocks(total, 256)), dim3(256), 0,
+                   M, N, nsplit, partial, alpha, beta, C, ldc, syrk ? 1 : 0);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void ptt_hip_dgemm(char ta, char tb, int M, int N, int K, double alpha, const double *A, long lda,
+                   const double *B, long ldb, double beta, double *C, long ldc) {
+    gemm_impl(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, false, false, "dgemm");
+}
+
+void ptt_hip_gram(char trans, int c, long len, const double *A, long lda, double *G) {
+    // SYRK skipping assumes one square tile grid (M == N); only the waves on
+    // or above the diagonal compute, the reduction mirrors.
+    bool syrk = true;
+    if (trans == 'T' || trans == 't')
+        gemm_impl('T', 'N', c, c, (int)len, 1.0, A, lda, A, lda, 0.0, G, c, syrk, true, "gram");
+    else
+        gemm_impl('N', 'T', c, c, (int)len, 1.0, A, lda, A, lda, 0.0, G, c, syrk, true, "gram");
+}
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+// runtime.hip -- device selection, the library stream, grow-only scratch
+// slots and HIP-event kernel profiling for the MI355X TT library.
+#include "common.hpp"
+
+#include <map>
+#include <string>
+#include <vector>
+
+namespace ptt {
+
+static bool g_init = false;
+hipStream_t g_stream = nullptr;
+
+void fatal_hip(hipError_t e, const char *what, const char *file, int line) {
+    fprintf(stderr, "paratt_amd: HIP error '%s' (%d) in %s at %s:%d\n",
+            hipGetErrorString(e), (int)e, what, file, line);
+    exit(1);
+}
+
+struct Slot {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+static Slot g_slots[PTT_WS_NSLOTS];
+
+// ---- profiling ----
+struct ProfRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+static bool g_prof = false;
+static std::vector<ProfRec> g_pending;
+static std::vector<hipEvent_t> g_free_events;
+struct ProfAcc {
+    int count = 0;
+    double ms = 0.0;
+};
+static std::map<std::string, ProfAcc> g_acc;
+
+static hipEvent_t get_event() {
+    if (!g_free_events.empty()) {
+        hipEvent_t e = g_free_events.back();
+        g_free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+}
+
+static void resolve_pending() {
+    if (g_pending.empty()) return;
+    HIPCHECK(hipStreamSynchronize(g_stream));
+    for (auto &r : g_pending) {
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, r.a, r.b));
+        auto &acc = g_acc[r.name];
+        acc.count += 1;
+        acc.ms += ms;
+        g_free_events.push_back(r.a);
+        g_free_events.push_back(r.b);
+    }
+    g_pending.clear();
+}
+
+ProfScope::ProfScope(const char *name) : active(g_prof), nm(name) {
+    if (!active) return;
+    a = get_event();
+    HIPCHECK(hipEventRecord(a, g_stream));
+}
+ProfScope::~ProfScope() {
+    if (!active) return;
+    hipEvent_t b = get_event();
+    HIPCHECK(hipEventRecord(b, g_stream));
+    g_pending.push_back({nm, a, b});
+    if (g_pending.size() > 4096) resolve_pending();
+}
+
+}  // namespace ptt
+
+using namespace ptt;
+
+extern "C" {
+
+void ptt_hip_init(void) {
+    if (g_init) return;
+    int ndev = 0;
+    HIPCHECK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) {
+        fprintf(stderr, "paratt_amd: no HIP device visible\n");
+        exit(1);
+    }
+    // Default-priority, non-blocking stream owned by the library.
+    HIPCHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_init = true;
+}
+
+void *ptt_hip_malloc(size_t bytes) {
+    ptt_hip_init();
+    void *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        fprintf(stderr, "paratt_amd: hipMalloc(%zu bytes) failed: %s\n", bytes,
+                hipGetErrorString(e));
+        exit(1);
+    }
+    return p;
+}
+
+void ptt_hip_free(void *p) {
+    if (!p) return;
+    HIPCHECK(hipStreamSynchronize(g_stream));
+    HIPCHECK(hipFree(p));
+}
+
+void ptt_hip_h2d(void *dst, const void *src, size_t bytes) {
+    ptt_hip_init();
+    if (!bytes) return;
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    HIPCHECK(hipStreamSynchronize(g_stream));
+}
+
+void ptt_hip_d2h(void *dst, const void *src, size_t bytes) {
+    ptt_hip_init();
+    if (!bytes) return;
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    HIPCHECK(hipStreamSynchronize(g_stream));
+}
+
+void ptt_hip_d2d(void *dst, const void *src, size_t bytes) {
+    ptt_hip_init();
+    if (!bytes || dst == src) return;
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+}
+
+void ptt_hip_sync(void) {
+    ptt_hip_init();
+    HIPCHECK(hipStreamSynchronize(g_stream));
+}
+
+void *ptt_hip_scratch(int slot, size_t bytes) {
+    ptt_hip_init();
+    if (slot < 0 || slot >= PTT_WS_NSLOTS) {
+        fprintf(stderr, "paratt_amd: bad scratch slot %d\n", slot);
+        exit(1);
+    }
+    Slot &s = g_slots[slot];
+    if (bytes > s.cap) {
+        if (s.p) ptt_hip_free(s.p);
+        size_t cap = bytes + bytes / 8 + 256;
+        s.p = ptt_hip_malloc(cap);
+        s.cap = cap;
+    }
+    return s.p;
+}
+
+size_t ptt_hip_scratch_cap(int slot) { return g_slots[slot].cap; }
+
+void ptt_hip_swap_tmp(int slot, double **buf, size_t *cap_bytes) {
+    Slot &s = g_slots[slot];
+    void *p = s.p;
+    size_t c = s.cap;
+    s.p = *buf;
+    s.cap = *cap_bytes;
+    *buf = (double *)p;
+    *cap_bytes = c;
+}
+
+void ptt_hip_prof_enable(int on) {
+    ptt_hip_init();
+    if (!on) resolve_pending();
+    g_prof = on != 0;
+}
+
+void ptt_hip_prof_reset(void) {
+    ptt_hip_init();
+    resolve_pending();
+    g_acc.clear();
+}
+
+int ptt_hip_prof_get(const char *name, int *count, double *total_ms) {
+    resolve_pending();
+    auto it = g_acc.find(name);
+    if (it == g_acc.end()) {
+        *count = 0;
+        *total_ms = 0.0;
+        return 0;
+    }
+    *count = it->second.count;
+    *total_ms = it->second.ms;
+    return 1;
+}
+
+void ptt_hip_prof_names(char *buf, int len) {
+    resolve_pending();
+    std::string s;
+    for (auto &kv : g_acc) {
+        s += kv.first;
+        s += '\n';
+    }
+    if (len <= 0) return;
+    size_t n = s.size() < (size_t)(len - 1) ? s.size() : (size_t)(len - 1);
+    memcpy(buf, s.data(), n);
+    buf[n] = 0;
+}
+
+static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+void ptt_hip_timer_start(void) {
+    ptt_hip_init();
+    if (!g_t0) {
+        HIPCHECK(hipEventCreate(&g_t0));
+        HIPCHECK(hipEventCreate(&g_t1));
+    }
+    HIPCHECK(hipEventRecord(g_t0, g_stream));
+}
+
+double ptt_hip_timer_stop(void) {
+    HIPCHECK(hipEventRecord(g_t1, g_stream));
+    HIPCHECK(hipEventSynchronize(g_t1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, g_t0, g_t1));
+    return (double)ms;
+}
+
+}  // extern "C"

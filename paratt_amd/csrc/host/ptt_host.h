@@ -1,0 +1,25 @@
+/* ptt_host.h -- internal helpers shared by the C host files. */
+#ifndef PARATT_AMD_PTT_HOST_H
+#define PARATT_AMD_PTT_HOST_H
+
+#include "paratt.h"
+
+/* Make core k's buffer the TMP scratch buffer (which holds the result of an
+ * out-of-place kernel) and give the old buffer to the scratch slot.  For a
+ * borrowed view (coreMem == NULL) the result is copied back instead, so the
+ * caller's pointers never change (`count` doubles of result). */
+void ptt_host_swap_core_with_tmp(attac_tt *x, int k, long count);
+/* Set the diagonal of an m x m device matrix to alpha. */
+void ptt_host_set_diag(double *A, int m, double alpha);
+/* Wrap raw (d, n, r, cores) arrays of the pattac* API as a borrowed TT. */
+void ptt_host_view(attac_tt *view, int d, int *n, int *r, double **x);
+void ptt_host_view_release(attac_tt *view, double **x);
+/* Ensure core k of x can hold `count` doubles (contents not preserved). */
+void ptt_host_reserve_core(attac_tt *x, int k, long count);
+/* Upload a small host coefficient vector to the COEF scratch slot. */
+double *ptt_host_coef(const double *host, long count);
+void ptt_host_fatal(const char *fn, const char *msg);
+/* paratt_truncate on a bare attac_tt. */
+void ptt_host_round(attac_tt *x, double tol, const int *maxRank);
+
+#endif

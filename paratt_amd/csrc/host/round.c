@@ -1,0 +1,444 @@
+/*
+ * round.c -- TT rounding on the GPU: left-to-right orthogonalisation and
+ * right-to-left SVD truncation of device-resident cores.
+ *
+ * Follows pattacTruncate (pattac.c:590-675): the reference scales every core
+ * by 1/sqrt(n_k), left-orthogonalises (pattac.c:244-294), truncates
+ * right-to-left with tolerance abs_tol/sqrt(d-1) per bond (pattac.c:297-522)
+ * and scales back by sqrt(n_k).  Here those scalings are folded into the
+ * alpha of the GEMMs that produce each core, so every intermediate has the
+ * reference's magnitude and no extra pass over the tensor is made.
+ *
+ * Factorizations (the MPI TSQR tree at one rank is a single dgeqrf/dgelqf,
+ * tsqr.c:541-714, 1048-1222) are Cholesky-QR on the matrix cores:
+ *   1 pass  when the Jacobi-scaled Gram is well conditioned (kappa <= 30):
+ *           orthogonality error ~ u*kappa^2 <= 1e-13;
+ *   2 passes (CholeskyQR2) when kappa <= 1e6;
+ *   3 passes (shifted CholeskyQR3, Fukaya et al., SISC 2020) otherwise,
+ *           which covers numerically rank-deficient cores.
+ * The bond SVD is a one-sided Jacobi SVD of the r x r triangular factor;
+ * in the well-conditioned case the new core is Sigma^-1 U^T times the old
+ * core (one GEMM, no explicit Q), otherwise V^T times the explicit Q.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "paratt.h"
+#include "paratt_amd.h"
+#include "ptt_hip.h"
+#include "ptt_host.h"
+
+#define KAPPA_ONE_PASS 30.0
+#define KAPPA_TWO_PASS 1.0e6
+
+static paratt_amd_round_stats g_stats;
+
+void paratt_amd_last_round_stats(paratt_amd_round_stats *out) { *out = g_stats; }
+
+/* Small device workspace for one step: matrices up to c x c. */
+typedef struct {
+    double *G, *R1, *T1, *R2, *T2, *R3, *T3, *Racc, *Rtmp, *L, *U, *V, *sig, *W, *US, *dinfo;
+    int *iinfo; /* [0] deficiency, [1..3] svd out */
+    int cap;
+} step_ws;
+
+static step_ws get_ws(int c) {
+    step_ws w;
+    size_t cc = (size_t)c * c;
+    double *base = (double *)ptt_hip_scratch(PTT_WS_SMALL, sizeof(double) * (15 * cc + 2 * (size_t)c + 64));
+    w.G = base;
+    w.R1 = w.G + cc;
+    w.T1 = w.R1 + cc;
+    w.R2 = w.T1 + cc;
+    w.T2 = w.R2 + cc;
+    w.R3 = w.T2 + cc;
+    w.T3 = w.R3 + cc;
+    w.Racc = w.T3 + cc;
+    w.Rtmp = w.Racc + cc;
+    w.L = w.Rtmp + cc;
+    w.U = w.L + cc;
+    w.V = w.U + cc;
+    w.W = w.V + cc;
+    w.US = w.W + cc;
+    w.sig = w.US + cc;
+    w.dinfo = w.sig + c + 8;
+    w.iinfo = (int *)ptt_hip_scratch(PTT_WS_INT, sizeof(int) * 64);
+    w.cap = c;
+    return w;
+}
+
+static double tiny_pivot(void) { return 1.0e-14; }
+
+/* Fukaya et al. shift for a Jacobi-scaled Gram of a len x c matrix. */
+static double cholqr_shift(long len, int c) {
+    double u = 0x1.0p-53;
+    return 11.0 * ((double)len * c + (double)c * (c + 1)) * u * (double)c;
+}
+
+typedef struct {
+    int deficient;
+    double kappa;
+} chol_out;
+
+static chol_out run_chol(int c, const double *G, double shift, double *R, double *T, step_ws *w) {
+    chol_out o;
+    ptt_hip_chol_qr_factor(c, G, shift, shift > 0 ? 0.0 : tiny_pivot(), R, T, w->iinfo, w->dinfo);
+    int info = 0;
+    double kap = 0.0;
+    ptt_hip_d2h(&info, w->iinfo, sizeof(int));
+    ptt_hip_d2h(&kap, w->dinfo, sizeof(double));
+    o.deficient = info;
+    o.kappa = kap;
+    return o;
+}
+
+/* In-place C = alpha * A * T (A: len x c, ld lda; T upper c x c).  Uses the
+ * TMP slot when the GEMM may not run in place (c > 128). */
+static void apply_right(attac_tt *x, int core, long len, int c, const double *T, double alpha) {
+    double *A = x->coreList[core];
+    if (c <= 128) {
+        ptt_hip_dgemm('N', 'N', (int)len, c, c, alpha, A, len, T, c, 0.0, A, len);
+    } else {
+        double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)len * c);
+        ptt_hip_dgemm('N', 'N', (int)len, c, c, alpha, A, len, T, c, 0.0, tmp, len);
+        ptt_host_swap_core_with_tmp(x, core, len * c);
+    }
+}
+
+/* In-place C = alpha * T^T * M (M: c x len, ld c; T upper c x c). */
+static void apply_left_t(attac_tt *x, int core, int c, long len, const double *T, double alpha) {
+    double *M = x->coreList[core];
+    if (c <= 128) {
+        ptt_hip_dgemm('T', 'N', c, (int)len, c, alpha, T, c, M, c, 0.0, M, c);
+    } else {
+        double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)len * c);
+        ptt_hip_dgemm('T', 'N', c, (int)len, c, alpha, T, c, M, c, 0.0, tmp, c);
+        ptt_host_swap_core_with_tmp(x, core, len * c);
+    }
+}
+
+/* Cholesky-QR of a core viewed as len x c (trans 'T': columns, LTR) or as
+ * c x len (trans 'N': rows, RTL).  On return the core holds Q (alpha
+ * applied) and R (c x c upper, A = Q*R resp. M = R^T*Q) is in *Rout.
+ * Returns the number of passes. */
+static int cholqr_core(attac_tt *x, int core, char trans, long len, int c, double qalpha,
+                       step_ws *w, const double **Rout, int allow_one_pass_noq, int *one_pass,
+                       const double **T1out) {
+    double *A = x->coreList[core];
+    ptt_hip_gram(trans, c, len, A, trans == 'T' ? len : c, w->G);
+    chol_out o = run_chol(c, w->G, 0.0, w->R1, w->T1, w);
+    if (one_pass) *one_pass = 0;
+    if (o.deficient == 0 && o.kappa <= KAPPA_ONE_PASS) {
+        if (allow_one_pass_noq) {
+            /* caller forms the new core itself from T1 (no explicit Q) */
+            *Rout = w->R1;
+            *T1out = w->T1;
+            if (one_pass) *one_pass = 1;
+            return 1;
+        }
+        if (trans == 'T') apply_right(x, core, len, c, w->T1, qalpha);
+        else apply_left_t(x, core, c, len, w->T1, qalpha);
+        *Rout = w->R1;
+        return 1;
+    }
+    double shift = 0.0;
+    int passes = 2;
+    if (!(o.deficient == 0 && o.kappa <= KAPPA_TWO_PASS)) {
+        shift = cholqr_shift(len, c);
+        o = run_chol(c, w->G, shift, w->R1, w->T1, w);
+        passes = 3;
+    }
+    /* pass 1 */
+    if (trans == 'T') apply_right(x, core, len, c, w->T1, 1.0);
+    else apply_left_t(x, core, c, len, w->T1, 1.0);
+    const double *Rprev = w->R1;
+    double *Racc = w->Racc;
+    ptt_hip_d2d(Racc, w->R1, sizeof(double) * (size_t)c * c);
+    for (int p = 2; p <= passes; p++) {
+        A = x->coreList[core];
+        ptt_hip_gram(trans, c, len, A, trans == 'T' ? len : c, w->G);
+        double *Rp = (p == 2) ? w->R2 : w->R3;
+        double *Tp = (p == 2) ? w->T2 : w->T3;
+        chol_out q = run_chol(c, w->G, 0.0, Rp, Tp, w);
+        if (q.deficient != 0 && p == passes) {
+            /* the last pass must not lose columns: shift it */
+            run_chol(c, w->G, cholqr_shift(len, c), Rp, Tp, w);
+        }
+        double a = (p == passes) ? qalpha : 1.0;
+        if (trans == 'T') apply_right(x, core, len, c, Tp, a);
+        else apply_left_t(x, core, c, len, Tp, a);
+        /* Racc <- Rp * Racc */
+        ptt_hip_trmm_small(c, Rp, Racc, w->Rtmp);
+        ptt_hip_d2d(Racc, w->Rtmp, sizeof(double) * (size_t)c * c);
+        (void)Rprev;
+    }
+    *Rout = Racc;
+    return passes;
+}
+
+/* Q = I_m (scaled) in core `core` viewed as m x m. */
+static void set_identity(double *dst, int m, double alpha) {
+    ptt_hip_zero(dst, sizeof(double) * (size_t)m * m);
+    ptt_host_set_diag(dst, m, alpha);
+}
+
+/* Left-to-right sweep over cores 0..d-2.  qscale[k]: alpha of core k's Q;
+ * rscale[k]: alpha applied when R_k multiplies core k+1. */
+static void ltr_sweep(attac_tt *x, const double *qscale, const double *rscale) {
+    int d = x->numDim;
+    int *r = x->ttRank, *n = x->dim;
+    for (int i = 0; i < d - 1; i++) {
+        long m = (long)r[i] * n[i];
+        int c = r[i + 1];
+        long wcols = (long)n[i + 1] * r[i + 2];
+        if (m < c) {
+            /* Q = I_m, R = A (m x c): core i+1 <- A * B (pattac.c:288's
+             * rank rule r_{i+1} <- min(r_i n_i, r_{i+1})) */
+            double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)m * wcols);
+            ptt_hip_dgemm('N', 'N', (int)m, (int)wcols, c, rscale[i], x->coreList[i], m,
+                          x->coreList[i + 1], c, 0.0, tmp, m);
+            ptt_host_swap_core_with_tmp(x, i + 1, m * wcols);
+            set_identity(x->coreList[i], (int)m, qscale[i]);
+            r[i + 1] = (int)m;
+            if (i < 64) g_stats.ltr_cholqr_passes[i] = 0;
+            continue;
+        }
+        step_ws w = get_ws(c);
+        const double *R = NULL;
+        int passes = cholqr_core(x, i, 'T', m, c, qscale[i], &w, &R, 0, NULL, NULL);
+        if (i < 64) g_stats.ltr_cholqr_passes[i] = passes;
+        /* core i+1 (c x wcols) <- rscale * R * core i+1 */
+        if (c <= 128) {
+            ptt_hip_dgemm('N', 'N', c, (int)wcols, c, rscale[i], R, c, x->coreList[i + 1], c, 0.0,
+                          x->coreList[i + 1], c);
+        } else {
+            double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)c * wcols);
+            ptt_hip_dgemm('N', 'N', c, (int)wcols, c, rscale[i], R, c, x->coreList[i + 1], c, 0.0,
+                          tmp, c);
+            ptt_host_swap_core_with_tmp(x, i + 1, (long)c * wcols);
+        }
+    }
+}
+
+/* Right-to-left truncation (pattac.c:297-522) of a left-orthogonal TT whose
+ * cores carry the reference's scaled magnitudes; tol is absolute in that
+ * scaled space.  nscale[k] = sqrt(n_k) is applied to each finished core. */
+static void rtl_sweep(attac_tt *x, const int *maxr, double abs_tol, const double *nscale) {
+    int d = x->numDim;
+    int *r = x->ttRank, *n = x->dim;
+    const double delta = (d > 1) ? abs_tol / sqrt(d - 1.0) : abs_tol;
+    for (int i = d - 1; i > 0; i--) {
+        int rows = r[i];
+        long w = (long)n[i] * r[i + 1];
+        long m1 = (long)r[i - 1] * n[i - 1];
+        double left_alpha = (i == 1) ? nscale[0] : 1.0;
+        int cap = maxr[i];
+        int rho;
+        step_ws ws;
+        if (w < rows) {
+            /* L has only w columns: SVD of the core itself (M = L*Q with Q = I_w) */
+            int k = (int)w;
+            ws = get_ws(rows > k ? rows : k);
+            ptt_hip_svd_jacobi(rows, k, x->coreList[i], rows, ws.sig, ws.U, rows, ws.V, k, 0.0,
+                               delta, cap, ws.iinfo + 1);
+            int io[3];
+            ptt_hip_d2h(io, ws.iinfo + 1, sizeof(int) * 3);
+            rho = io[1];
+            if (i < 64) { g_stats.svd_sweeps[i] = io[2]; g_stats.numrank[i] = io[0]; g_stats.rtl_cholqr_passes[i] = 0; }
+            /* core i <- sqrt(n_i) * V_rho^T (rho x w) */
+            ptt_hip_transpose(k, rho, ws.V, k, x->coreList[i], rho, nscale[i]);
+            ptt_hip_svd_factors(rows, rho, ws.iinfo + 2, ws.U, rows, ws.sig, 1.0, NULL, ws.US);
+        } else {
+            ws = get_ws(rows);
+            const double *R = NULL, *T1 = NULL;
+            int one_pass = 0;
+            int passes = cholqr_core(x, i, 'N', w, rows, 1.0, &ws, &R, 1, &one_pass, &T1);
+            if (i < 64) g_stats.rtl_cholqr_passes[i] = passes;
+            /* L = R^T (rows x rows, lower) */
+            ptt_hip_transpose(rows, rows, R, rows, ws.L, rows, 1.0);
+            ptt_hip_svd_jacobi(rows, rows, ws.L, rows, ws.sig, ws.U, rows, one_pass ? NULL : ws.V,
+                               rows, 0.0, delta, cap, ws.iinfo + 1);
+            int io[3];
+            ptt_hip_d2h(io, ws.iinfo + 1, sizeof(int) * 3);
+            rho = io[1];
+            if (i < 64) { g_stats.svd_sweeps[i] = io[2]; g_stats.numrank[i] = io[0]; }
+            double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)rho * w);
+            if (one_pass) {
+                /* new core = sqrt(n_i) Sigma^-1 U_rho^T M  (rho x w) */
+                ptt_hip_svd_factors(rows, rho, ws.iinfo + 2, ws.U, rows, ws.sig, nscale[i], ws.W,
+                                    ws.US);
+                ptt_hip_dgemm('N', 'N', rho, (int)w, rows, 1.0, ws.W, rho, x->coreList[i], rows,
+                              0.0, tmp, rho);
+            } else {
+                /* new core = sqrt(n_i) V_rho^T Q */
+                ptt_hip_svd_factors(rows, rho, ws.iinfo + 2, ws.U, rows, ws.sig, 1.0, NULL, ws.US);
+                ptt_hip_dgemm('T', 'N', rho, (int)w, rows, nscale[i], ws.V, rows, x->coreList[i],
+                              rows, 0.0, tmp, rho);
+            }
+            ptt_host_swap_core_with_tmp(x, i, (long)rho * w);
+        }
+        if (i < 64) {
+            double sv[4] = {0, 0, 0, 0};
+            int k = (int)(w < rows ? w : rows);
+            int lo = rho - 1, cnt = 0;
+            for (int q = lo; q < lo + 4 && q < k; q++) cnt++;
+            if (cnt > 0) ptt_hip_d2h(sv, ws.sig + lo, sizeof(double) * cnt);
+            for (int q = 0; q < 4; q++) g_stats.sig[i][q] = sv[q];
+        }
+        /* core i-1 (m1 x rows) <- left_alpha * core i-1 * U_rho Sigma_rho */
+        if (rho <= 128) {
+            ptt_hip_dgemm('N', 'N', (int)m1, rho, rows, left_alpha, x->coreList[i - 1], m1, ws.US,
+                          rows, 0.0, x->coreList[i - 1], m1);
+        } else {
+            double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)m1 * rho);
+            ptt_hip_dgemm('N', 'N', (int)m1, rho, rows, left_alpha, x->coreList[i - 1], m1, ws.US,
+                          rows, 0.0, tmp, m1);
+            ptt_host_swap_core_with_tmp(x, i - 1, m1 * rho);
+        }
+        r[i] = rho;
+    }
+}
+
+static void fill_scales(const attac_tt *x, double *inv, double *sq) {
+    for (int k = 0; k < x->numDim; k++) {
+        sq[k] = sqrt((double)x->dim[k]);
+        inv[k] = 1.0 / sq[k];
+    }
+}
+
+/* Shared driver: returns ||x||_n from the sweep (last core's Frobenius norm
+ * in the scaled space). */
+static double round_impl(attac_tt *x, const int *maxRank, double tol, int tol_is_rel) {
+    int d = x->numDim;
+    memset(&g_stats, 0, sizeof g_stats);
+    g_stats.d = d;
+    int *mr = (int *)malloc(sizeof(int) * (d + 1));
+    memcpy(mr, maxRank, sizeof(int) * (d + 1)); /* maxRank may alias ttRank */
+    double *inv = (double *)malloc(sizeof(double) * d);
+    double *sq = (double *)malloc(sizeof(double) * d);
+    double *qs = (double *)malloc(sizeof(double) * d);
+    double *rs = (double *)malloc(sizeof(double) * d);
+    fill_scales(x, inv, sq);
+    for (int k = 0; k < d; k++) {
+        qs[k] = 1.0;
+        rs[k] = (k + 1 < d) ? inv[k + 1] : 1.0;
+    }
+    if (d >= 2) rs[0] *= inv[0];
+    double nrm = 0.0;
+    if (d >= 2) {
+        ltr_sweep(x, qs, rs);
+        /* norm in the scaled space: last core after the sweep */
+        double *dss = (double *)ptt_hip_scratch(PTT_WS_INT, sizeof(int) * 64) + 24;
+        ptt_hip_sumsq(x->ttRank[d - 1], x->dim[d - 1], x->coreList[d - 1], x->ttRank[d - 1], dss);
+        double ss = 0.0;
+        ptt_hip_d2h(&ss, dss, sizeof(double));
+        nrm = sqrt(ss);
+        double abs_tol = tol_is_rel ? tol * nrm : tol;
+        rtl_sweep(x, mr, abs_tol, sq);
+    } else {
+        /* one core: scale, nothing to orthogonalise or truncate, unscale */
+        long cnt = (long)x->ttRank[0] * x->dim[0] * x->ttRank[1];
+        double ss = 0.0;
+        double *dss = (double *)ptt_hip_scratch(PTT_WS_INT, sizeof(int) * 64) + 24;
+        ptt_hip_sumsq(1, (int)cnt, x->coreList[0], 1, dss);
+        ptt_hip_d2h(&ss, dss, sizeof(double));
+        nrm = sqrt(ss) * inv[0];
+    }
+    g_stats.norm = nrm;
+    ptt_hip_sync();
+    free(mr); free(inv); free(sq); free(qs); free(rs);
+    return nrm;
+}
+
+/* ---- public entry points ---- */
+
+void paratt_truncate(paratt *x_ptt, double tol, int *maxRank) {
+    round_impl(x_ptt->x, maxRank, tol, 0);
+}
+
+void paratt_amd_truncate_rel(paratt *x_ptt, double eps_rel, int *maxRank) {
+    round_impl(x_ptt->x, maxRank, eps_rel, 1);
+}
+
+void pattacTruncate(MPI_Comm comm, const int d, int *distrib_dim_mat, int *rx, double **x,
+                    const int *maxr, const double abs_tol, const double rel_tol, const char ortho,
+                    tsqr_tree *qrTree, int *int_wrk, double **flt_wrk_ptr, int *flt_sz_ptr) {
+    (void)comm; (void)rel_tol; (void)qrTree; (void)int_wrk; (void)flt_wrk_ptr; (void)flt_sz_ptr;
+    if (ortho != 'R') {
+        fprintf(stderr, "%s:Line %d %s::Not known orthogonalization side\n", __FILE__, __LINE__,
+                __func__);
+        exit(1);
+    }
+    attac_tt view;
+    ptt_host_view(&view, d, distrib_dim_mat, rx, x);
+    round_impl(&view, maxr, abs_tol, 0);
+    ptt_host_view_release(&view, x);
+}
+
+void pattacLOrthogonalization(MPI_Comm comm, int d, int *distrib_dim_mat, int *rx, double **x,
+                              tsqr_tree *qrTree, int *int_wrk) {
+    (void)comm; (void)qrTree; (void)int_wrk;
+    attac_tt view;
+    ptt_host_view(&view, d, distrib_dim_mat, rx, x);
+    double *qs = (double *)malloc(sizeof(double) * d);
+    double *rs = (double *)malloc(sizeof(double) * d);
+    for (int k = 0; k < d; k++) { qs[k] = 1.0; rs[k] = 1.0; }
+    if (d >= 2) ltr_sweep(&view, qs, rs);
+    ptt_hip_sync();
+    ptt_host_view_release(&view, x);
+    free(qs); free(rs);
+}
+
+void pattacLTruncate(MPI_Comm comm, const int d, int *distrib_dim_mat, int *rx, double **x,
+                     const int *maxr, const double abs_tol, tsqr_tree *qrTree, int *int_wrk,
+                     double **flt_wrk_ptr, int *flt_sz_ptr) {
+    (void)comm; (void)qrTree; (void)int_wrk; (void)flt_wrk_ptr; (void)flt_sz_ptr;
+    attac_tt view;
+    ptt_host_view(&view, d, distrib_dim_mat, rx, x);
+    double *ones = (double *)malloc(sizeof(double) * d);
+    for (int k = 0; k < d; k++) ones[k] = 1.0;
+    int *mr = (int *)malloc(sizeof(int) * (d + 1));
+    memcpy(mr, maxr, sizeof(int) * (d + 1));
+    if (d >= 2) rtl_sweep(&view, mr, abs_tol, ones);
+    ptt_hip_sync();
+    ptt_host_view_release(&view, x);
+    free(ones); free(mr);
+}
+
+double paratt_norm(paratt *x_ptt) {
+    attac_tt *x = x_ptt->x;
+    int d = x->numDim;
+    double *inv = (double *)malloc(sizeof(double) * d);
+    double *sq = (double *)malloc(sizeof(double) * d);
+    double *qs = (double *)malloc(sizeof(double) * d);
+    double *rs = (double *)malloc(sizeof(double) * d);
+    fill_scales(x, inv, sq);
+    for (int k = 0; k < d; k++) {
+        qs[k] = sq[k];                       /* leaves core k = sqrt(n_k) * Q_k */
+        rs[k] = (k + 1 < d) ? inv[k + 1] : 1.0;
+    }
+    if (d >= 2) rs[0] *= inv[0];
+    double *dss = (double *)ptt_hip_scratch(PTT_WS_INT, sizeof(int) * 64) + 24;
+    double ss = 0.0;
+    if (d >= 2) {
+        ltr_sweep(x, qs, rs);
+        int rl = x->ttRank[d - 1];
+        ptt_hip_sumsq(rl, x->dim[d - 1], x->coreList[d - 1], rl, dss);
+        ptt_hip_d2h(&ss, dss, sizeof(double));
+        /* reference leaves the last core unscaled by sqrt(n_{d-1}) */
+        ptt_hip_scale(x->coreList[d - 1], (long)rl * x->dim[d - 1], sq[d - 1]);
+    } else {
+        long cnt = (long)x->ttRank[0] * x->dim[0] * x->ttRank[1];
+        ptt_hip_sumsq(1, (int)cnt, x->coreList[0], 1, dss);
+        ptt_hip_d2h(&ss, dss, sizeof(double));
+        ss *= inv[0] * inv[0];
+    }
+    ptt_hip_sync();
+    free(inv); free(sq); free(qs); free(rs);
+    return sqrt(ss);
+}
+
+/* Rounding of a bare attac_tt (used by attac_exp_euler_st). */
+void ptt_host_round(attac_tt *x, double tol, const int *maxRank) { round_impl(x, maxRank, tol, 0); }

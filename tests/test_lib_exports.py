@@ -1,0 +1,51 @@
+"""CPU-side checks of the C-ABI library: it is built, loads, and exports
+every entry point the public headers declare (no device calls here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import paratt_amd as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ["paratt.h", "paratt_ode.h", "pattac.h", "paratt_amd.h", "ptt_hip.h"]:
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        src = "\n".join(l for l in src.split("\n") if not l.lstrip().startswith("#"))
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", src):
+            name = m.group(1)
+            if name in ("if", "for", "while", "sizeof", "return", "defined", "void", "int",
+                        "double", "char", "MPI_Comm", "size_t", "uint64_t"):
+                continue
+            pre = src[: m.start()].rstrip()
+            if pre.endswith("(*") or pre.endswith("typedef"):
+                continue
+            names.add(name)
+    return sorted(names)
+
+
+def test_library_is_built():
+    assert os.path.exists(P.LIB_PATH), "run __graft_entry__.build() or make -C paratt_amd"
+
+
+@pytest.mark.parametrize("name", declared_functions())
+def test_header_symbol_exported(name):
+    lib = C.CDLL(P.LIB_PATH)
+    assert hasattr(lib, name), name
+
+
+def test_public_symbol_list_exported():
+    lib = C.CDLL(P.LIB_PATH)
+    missing = [s for s in P.PUBLIC_SYMBOLS if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_reference_headers():
+    # attac_tt: int, 4 pointers + int, padded; paratt: pointer-first layout
+    assert C.sizeof(P.AttacTT) == 56
+    assert C.sizeof(P.Paratt) == 64
