@@ -120,7 +120,9 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, int kchunk,
     }
     if (!active) return;
 
-    // acc[i][j][v] = C[m = m0+wm*WM+i*16+(lane&15)][n = n0+wn*WN+j*16+4*(lane>>4)+v]
+    // v_mfma_f64_16x16x4f64 output: lane l, register v holds D[l/16 + 4v][l%16]
+    // (checked on gfx950 with a lane-layout probe).  With D = C^T tile:
+    // acc[i][j][v] = C[m = m0+wm*WM+i*16+(lane&15)][n = n0+wn*WN+j*16+(lane>>4)+4v]
 #pragma unroll
     for (int i = 0; i < TM; i++) {
         const int gm = m0 + wm * WM + i * 16 + (lane & 15);
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, int kchunk,
         for (int j = 0; j < TN; j++) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int gn = n0 + wn * WN + j * 16 + 4 * (lane >> 4) + v;
+                const int gn = n0 + wn * WN + j * 16 + (lane >> 4) + 4 * v;
                 if (gn >= N) continue;
                 double r = acc[i][j][v];
                 if (partial) {
