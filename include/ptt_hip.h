@@ -43,6 +43,8 @@ void ptt_hip_prof_enable(int on);
 void ptt_hip_prof_reset(void);
 /* Returns 1 and fills count / total_ms if `name` was recorded (syncs). */
 int ptt_hip_prof_get(const char *name, int *count, double *total_ms);
+/* Algorithmic flops and bytes credited to `name` (syncs). */
+int ptt_hip_prof_get_work(const char *name, double *flops, double *bytes);
 /* Names recorded so far, '\n'-separated, into buf (truncated to len). */
 void ptt_hip_prof_names(char *buf, int len);
 /* Wall-clock of the stream between two markers, in ms (syncs). */

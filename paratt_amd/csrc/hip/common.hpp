@@ -26,18 +26,23 @@ extern hipStream_t g_stream;
 struct ProfScope {
     bool active;
     const char *nm;
+    double flops, bytes;
     hipEvent_t a;
-    explicit ProfScope(const char *name);
+    explicit ProfScope(const char *name, double flops = 0.0, double bytes = 0.0);
     ~ProfScope();
 };
 
-#define PTT_LAUNCH(name, kern, grid, block, shmem, ...)                 \
+// Launch on the library stream; with profiling on, HIP events bracket the
+// launch and its algorithmic flops / bytes are credited to `name`.
+#define PTT_LAUNCH_FB(name, flops, bytes, kern, grid, block, shmem, ...) \
     do {                                                                \
-        ptt::ProfScope ps_(name);                                       \
+        ptt::ProfScope ps_(name, flops, bytes);                         \
         hipLaunchKernelGGL(kern, grid, block, shmem, ptt::g_stream,     \
                            __VA_ARGS__);                                \
         HIPCHECK(hipGetLastError());                                    \
     } while (0)
+#define PTT_LAUNCH(name, kern, grid, block, shmem, ...) \
+    PTT_LAUNCH_FB(name, 0.0, 0.0, kern, grid, block, shmem, __VA_ARGS__)
 
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 

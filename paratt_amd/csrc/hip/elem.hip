@@ -316,7 +316,8 @@ void ptt_hip_axpby_core(int kind, int n, int rx0, int rx1, const double *x, int 
     ptt_hip_init();
     long rz0 = (kind == 0) ? 1 : rx0 + ry0, rz1 = (kind == 2) ? 1 : rx1 + ry1;
     long total = rz0 * n * rz1;
-    PTT_LAUNCH("axpby", k_axpby_core, dim3(stream_blocks(total, 256)), dim3(256), 0, kind, n,
+    double bytes = 8.0 * ((double)rx0 * n * rx1 + (double)ry0 * n * ry1 + (double)total);
+    PTT_LAUNCH_FB("axpby", 0.0, bytes, k_axpby_core, dim3(stream_blocks(total, 256)), dim3(256), 0, kind, n,
                rx0, rx1, x, ry0, ry1, y, a, b, z);
 }
 
@@ -324,14 +325,15 @@ void ptt_hip_hadamard_core(int n, int rx0, int rx1, const double *x, int ry0, in
                            const double *y, double *z) {
     ptt_hip_init();
     long total = (long)rx0 * ry0 * n * rx1 * ry1;
-    PTT_LAUNCH("hadamard", k_hadamard_core, dim3(stream_blocks(total, 256)), dim3(256), 0, n,
+    double bytes = 8.0 * ((double)rx0 * n * rx1 + (double)ry0 * n * ry1 + (double)total);
+    PTT_LAUNCH_FB("hadamard", 0.0, bytes, k_hadamard_core, dim3(stream_blocks(total, 256)), dim3(256), 0, n,
                rx0, rx1, x, ry0, ry1, y, z);
 }
 
 void ptt_hip_diag_core(int r0, int n, int r1, const double *v, double *f) {
     ptt_hip_init();
     long total = (long)r0 * n * r1;
-    PTT_LAUNCH("diag", k_diag_core, dim3(stream_blocks(total, 256)), dim3(256), 0, r0, n, r1, v,
+    PTT_LAUNCH_FB("diag", (double)total, 16.0 * total, k_diag_core, dim3(stream_blocks(total, 256)), dim3(256), 0, r0, n, r1, v,
                f);
 }
 
@@ -339,7 +341,7 @@ void ptt_hip_stencil_core(int kind, int r0, int n, int r1, const double *sten, i
                           const double *f, double *g) {
     ptt_hip_init();
     long total = (long)r0 * n * r1;
-    PTT_LAUNCH("stencil", k_stencil_core, dim3(stream_blocks(total, 256)), dim3(256), 0, kind,
+    PTT_LAUNCH_FB("stencil", 2.0 * slen * total, 16.0 * total, k_stencil_core, dim3(stream_blocks(total, 256)), dim3(256), 0, kind,
                r0, n, r1, sten, slen, f, g);
 }
 

@@ -159,12 +159,32 @@ __global__ void k_splitk_reduce(int M, int N, int nsplit, const double *__restri
     }
 }
 
+// Profile name per template instantiation, e.g. "dgemm_TN_128x128_syrk",
+// so event timings line up with rocprofv3's per-kernel statistics.
+template <int BM, int BN, bool TA, bool TB, bool SYRK>
+const char *inst_name() {
+    static char buf[48];
+    static bool init = false;
+    if (!init) {
+        snprintf(buf, sizeof buf, "dgemm_%c%c_%dx%d%s", TA ? 'T' : 'N', TB ? 'T' : 'N', BM, BN,
+                 SYRK ? "_syrk" : "");
+        init = true;
+    }
+    return buf;
+}
+
 template <int BM, int BN, bool TA, bool TB, bool SYRK>
 void launch_t(dim3 grid, int M, int N, int K, int kchunk, double alpha, const double *A,
               long lda, const double *B, long ldb, double beta, double *C, long ldc,
               double *partial, const char *name) {
-    PTT_LAUNCH(name, (k_dgemm<BM, BN, TA, TB, SYRK>), grid, dim3(256), 0, M, N, K, kchunk,
-               alpha, A, lda, B, ldb, beta, C, ldc, partial);
+    (void)name;
+    // algorithmic work: 2*M*N*K flops (the Gram counts its full square);
+    // bytes: operands read once, result written once
+    double flops = 2.0 * M * N * (double)K;
+    double bytes = 8.0 * ((double)M * K + (double)K * N + (double)M * N);
+    PTT_LAUNCH_FB((inst_name<BM, BN, TA, TB, SYRK>()), flops, bytes,
+                  (k_dgemm<BM, BN, TA, TB, SYRK>), grid, dim3(256), 0, M, N, K, kchunk,
+                  alpha, A, lda, B, ldb, beta, C, ldc, partial);
 }
 
 template <int BM, int BN, bool SYRK>

@@ -27,13 +27,14 @@ static Slot g_slots[PTT_WS_NSLOTS];
 struct ProfRec {
     std::string name;
     hipEvent_t a, b;
+    double flops, bytes;
 };
 static bool g_prof = false;
 static std::vector<ProfRec> g_pending;
 static std::vector<hipEvent_t> g_free_events;
 struct ProfAcc {
     int count = 0;
-    double ms = 0.0;
+    double ms = 0.0, flops = 0.0, bytes = 0.0;
 };
 static std::map<std::string, ProfAcc> g_acc;
 
@@ -57,13 +58,16 @@ static void resolve_pending() {
         auto &acc = g_acc[r.name];
         acc.count += 1;
         acc.ms += ms;
+        acc.flops += r.flops;
+        acc.bytes += r.bytes;
         g_free_events.push_back(r.a);
         g_free_events.push_back(r.b);
     }
     g_pending.clear();
 }
 
-ProfScope::ProfScope(const char *name) : active(g_prof), nm(name) {
+ProfScope::ProfScope(const char *name, double f, double b)
+    : active(g_prof), nm(name), flops(f), bytes(b) {
     if (!active) return;
     a = get_event();
     HIPCHECK(hipEventRecord(a, g_stream));
@@ -72,7 +76,7 @@ ProfScope::~ProfScope() {
     if (!active) return;
     hipEvent_t b = get_event();
     HIPCHECK(hipEventRecord(b, g_stream));
-    g_pending.push_back({nm, a, b});
+    g_pending.push_back({nm, a, b, flops, bytes});
     if (g_pending.size() > 4096) resolve_pending();
 }
 
@@ -189,6 +193,19 @@ int ptt_hip_prof_get(const char *name, int *count, double *total_ms) {
     }
     *count = it->second.count;
     *total_ms = it->second.ms;
+    return 1;
+}
+
+int ptt_hip_prof_get_work(const char *name, double *flops, double *bytes) {
+    resolve_pending();
+    auto it = g_acc.find(name);
+    if (it == g_acc.end()) {
+        *flops = 0.0;
+        *bytes = 0.0;
+        return 0;
+    }
+    *flops = it->second.flops;
+    *bytes = it->second.bytes;
     return 1;
 }
 
