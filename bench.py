@@ -193,6 +193,8 @@ def main():
         "svd_sweeps": list(stats.svd_sweeps[1:d]),
         "cholqr_passes_ltr": sorted(set(stats.ltr_cholqr_passes[:d - 1])),
         "cholqr_passes_rtl": sorted(set(stats.rtl_cholqr_passes[1:d])),
+        "kappa_ltr_max": round(max(stats.kappa_ltr[:d - 1]), 3),
+        "kappa_rtl_max": round(max(stats.kappa_rtl[1:d]), 3),
     }
     print(json.dumps(line))
     sys.stdout.flush()

@@ -38,6 +38,8 @@ typedef struct {
     int numrank[64];             /* numerical rank per bond */
     double norm;                 /* ||x||_n computed by the sweep */
     double sig[64][8];           /* sigma[rho-1..rho+2] around each cut (scaled) */
+    double kappa_ltr[64];        /* condition estimate of each column-scaled core */
+    double kappa_rtl[64];
 } paratt_amd_round_stats;
 void paratt_amd_last_round_stats(paratt_amd_round_stats *out);
 

@@ -85,6 +85,12 @@ void ptt_hip_copy2d(int m, int n, const double *A, long lda, double *B, long ldb
 /* B (n x m) = alpha * A^T (A m x n). */
 void ptt_hip_transpose(int m, int n, const double *A, long lda, double *B,
                        long ldb, double alpha);
+/* Place src (sr0 x n x sr1) at row/col offset (ra, rb) of dst
+ * (dr0 x n x dr1), scaled by alpha: dst[ra+a, j, rb+b] = alpha*src[a, j, b]. */
+void ptt_hip_block_place(double *dst, int dr0, int n, int dr1, int ra, int rb,
+                         const double *src, int sr0, int sr1, double alpha);
+/* p[0..count) = v. */
+void ptt_hip_fill(double *p, long count, double v);
 /* A[i + i*lda] = alpha for i < m (other entries untouched). */
 void ptt_hip_set_diag(double *A, int m, long lda, double alpha);
 /* Squared Frobenius norm of an m x n block, result to device *out. */

@@ -53,7 +53,8 @@ class RoundStats(C.Structure):
     _fields_ = [("d", C.c_int), ("ltr_cholqr_passes", C.c_int * 64),
                 ("rtl_cholqr_passes", C.c_int * 64), ("svd_sweeps", C.c_int * 64),
                 ("numrank", C.c_int * 64), ("norm", C.c_double),
-                ("sig", (C.c_double * 8) * 64)]
+                ("sig", (C.c_double * 8) * 64), ("kappa_ltr", C.c_double * 64),
+                ("kappa_rtl", C.c_double * 64)]
 
 
 _lib = None

@@ -22,6 +22,10 @@ namespace {
 constexpr int NT = 1024;
 constexpr int LDS_DOUBLES = 17408;  // 136 KB dynamic + static arrays <= 160 KB LDS
 
+// Work matrix S is stored ROW-major in LDS (S[i*c + k] = entry (i,k)) so the
+// pivot row read by every thread in the trailing update is contiguous and
+// each thread's own entries are consecutive across lanes: no bank conflicts.
+template <bool INLDS>
 __global__ __launch_bounds__(NT) void k_chol_qr_factor(int c, const double *__restrict__ G,
                                                        double shift, double tiny,
                                                        double *__restrict__ R,
@@ -29,110 +33,183 @@ __global__ __launch_bounds__(NT) void k_chol_qr_factor(int c, const double *__re
                                                        double *dinfo, double *gwork) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
-    // Work matrix S (c x c): LDS if it fits, else global scratch.
-    double *S = (c * c + 3 * c <= LDS_DOUBLES) ? lds : gwork;
-    double *D = (c * c + 3 * c <= LDS_DOUBLES) ? lds + c * c : gwork + (long)c * c;
-    double *col = D + c;   // temp column
-    double *dd = col + c;  // pivots (as double)
-    __shared__ int s_def[1];
+    // INLDS: the work matrix is LDS-resident (the compiler sees the shared
+    // address space and issues ds_* instructions); else global scratch.
+    double *S = INLDS ? lds : gwork;
+    double *D = S + (long)c * c;
+    double *col = D + c;   // temp vector
+    double *dd = col + c;  // deficient flags
+    double *pv = dd + c;   // power-iteration vector (2c)
+    __shared__ int s_def;
     __shared__ double s_red[NT / 64];
-    __shared__ double s_rjj;
+    __shared__ double s_rjj, s_scal;
     __shared__ int s_isdef;
 
     for (int j = tid; j < c; j += NT) {
         double g = G[j + (long)j * c];
         D[j] = g > 0.0 ? sqrt(g) : 0.0;
     }
-    if (tid == 0) s_def[0] = 0;
+    if (tid == 0) s_def = 0;
     __syncthreads();
+    // G is symmetric: S(i,k) = G[i + k*c] / (D_i D_k), stored row-major.
     for (long e = tid; e < (long)c * c; e += NT) {
-        int i = (int)(e % c), k = (int)(e / c);
+        int k = (int)(e % c), i = (int)(e / c);
         double di = D[i], dk = D[k];
-        S[e] = (di > 0.0 && dk > 0.0) ? G[e] / (di * dk) : 0.0;
+        S[e] = (di > 0.0 && dk > 0.0) ? G[k + (long)i * c] / (di * dk) : 0.0;
     }
     __syncthreads();
-    // Right-looking Cholesky on the upper triangle: S[i + c*k], i <= k.
+    // lambda_max of the scaled Gram by power iteration (before it is overwritten)
+    auto block_sum = [&](double v) -> double {
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        __syncthreads();
+        if ((tid & 63) == 0) s_red[tid >> 6] = v;
+        __syncthreads();
+        double t = 0.0;
+        for (int i = 0; i < NT / 64; i++) t += s_red[i];
+        return t;
+    };
+    const int PIT = 12;
+    for (int i = tid; i < c; i += NT) pv[i] = 1.0 + 0.01 * (i % 7);
+    __syncthreads();
+    double lmax = 0.0;
+    for (int it = 0; it < PIT; it++) {
+        // y = S x (rows in parallel, 8 lanes per row)
+        const int row = tid >> 3, sub = tid & 7;
+        for (int i0 = 0; i0 < c; i0 += NT / 8) {
+            int i = i0 + row;
+            double acc = 0.0;
+            if (i < c)
+                for (int l = sub; l < c; l += 8) acc += S[(long)i * c + l] * pv[l];
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 2);
+            acc += __shfl_xor(acc, 4);
+            if (i < c && sub == 0) pv[c + i] = acc;
+        }
+        __syncthreads();
+        double nn = 0.0;
+        for (int i = tid; i < c; i += NT) nn += pv[c + i] * pv[c + i];
+        nn = block_sum(nn);
+        double xn = 0.0;
+        for (int i = tid; i < c; i += NT) xn += pv[i] * pv[i];
+        xn = block_sum(xn);
+        lmax = (xn > 0.0) ? sqrt(nn / xn) : 0.0;
+        double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+        for (int i = tid; i < c; i += NT) pv[i] = pv[c + i] * inv;
+        __syncthreads();
+    }
+    // Right-looking Cholesky, upper triangle, row-major.
     for (int j = 0; j < c; j++) {
         if (tid == 0) {
-            double p = S[j + (long)j * c] + shift;
+            double p = S[(long)j * c + j] + shift;
             int dead = !(D[j] > 0.0) || !(p > tiny);
             s_isdef = dead;
             s_rjj = dead ? 0.0 : sqrt(p);
-            S[j + (long)j * c] = s_rjj;
+            S[(long)j * c + j] = s_rjj;
             dd[j] = dead ? 1.0 : 0.0;
-            if (dead) s_def[0] += 1;
+            if (dead) s_def += 1;
         }
         __syncthreads();
         const double rjj = s_rjj;
         const int dead = s_isdef;
         for (int k = j + 1 + tid; k < c; k += NT)
-            S[j + (long)k * c] = dead ? 0.0 : S[j + (long)k * c] / rjj;
+            S[(long)j * c + k] = dead ? 0.0 : S[(long)j * c + k] / rjj;
         __syncthreads();
         if (!dead) {
             const int w = c - j - 1;
+            // entries (i,k), j < i <= k: thread -> consecutive k for fixed i
             for (long t = tid; t < (long)w * w; t += NT) {
-                int i = j + 1 + (int)(t % w), k = j + 1 + (int)(t / w);
-                if (i <= k) S[i + (long)k * c] -= S[j + (long)i * c] * S[j + (long)k * c];
+                int i = j + 1 + (int)(t / w), k = j + 1 + (int)(t % w);
+                if (i <= k) S[(long)i * c + k] -= S[(long)j * c + i] * S[(long)j * c + k];
             }
         }
         __syncthreads();
     }
-    // ||Rs||_F and R = Rs * D (upper, zeros below).
+    // ||Rs||_F and R = Rs * D (column-major output, zeros below).
     double fr = 0.0;
     for (long e = tid; e < (long)c * c; e += NT) {
         int i = (int)(e % c), k = (int)(e / c);
-        double v = (i <= k) ? S[e] : 0.0;
+        double v = (i <= k) ? S[(long)i * c + k] : 0.0;
         fr += v * v;
         R[e] = v * D[k];
     }
-    for (int o = 32; o > 0; o >>= 1) fr += __shfl_xor(fr, o);
-    if ((tid & 63) == 0) s_red[tid >> 6] = fr;
-    __syncthreads();
-    if (tid == 0) {
-        double t = 0.0;
-        for (int i = 0; i < NT / 64; i++) t += s_red[i];
-        dinfo[0] = sqrt(t);
-    }
-    __syncthreads();
-    // In-place inverse of the upper triangle, column by column; 8 lanes per
-    // row share each dot product.
+    fr = block_sum(fr);
+    // In-place inverse of the upper triangle, column by column (row-major S:
+    // row i of the inverse is contiguous); 8 lanes per row share each dot.
     for (int j = 0; j < c; j++) {
         const bool dead = dd[j] != 0.0;
-        for (int i = tid; i < j; i += NT) col[i] = S[i + (long)j * c];
+        for (int i = tid; i < j; i += NT) col[i] = S[(long)i * c + j];
         __syncthreads();
-        const double tjj = dead ? 0.0 : 1.0 / S[j + (long)j * c];
+        const double tjj = dead ? 0.0 : 1.0 / S[(long)j * c + j];
         const int row = tid >> 3, sub = tid & 7;
         for (int i0 = 0; i0 < j; i0 += NT / 8) {
             int i = i0 + row;
-            double s = 0.0;
+            double sacc = 0.0;
             if (i < j && !dead)
-                for (int l = i + sub; l < j; l += 8) s += S[i + (long)l * c] * col[l];
-            s += __shfl_xor(s, 1);
-            s += __shfl_xor(s, 2);
-            s += __shfl_xor(s, 4);
-            if (i < j && sub == 0) S[i + (long)j * c] = dead ? 0.0 : -s * tjj;
+                for (int l = i + sub; l < j; l += 8) sacc += S[(long)i * c + l] * col[l];
+            sacc += __shfl_xor(sacc, 1);
+            sacc += __shfl_xor(sacc, 2);
+            sacc += __shfl_xor(sacc, 4);
+            if (i < j && sub == 0) S[(long)i * c + j] = dead ? 0.0 : -sacc * tjj;
         }
         __syncthreads();
-        if (tid == 0) S[j + (long)j * c] = tjj;
+        if (tid == 0) S[(long)j * c + j] = tjj;
         __syncthreads();
     }
     double ft = 0.0;
     for (long e = tid; e < (long)c * c; e += NT) {
         int i = (int)(e % c), k = (int)(e / c);
-        double v = (i <= k) ? S[e] : 0.0;
+        double v = (i <= k) ? S[(long)i * c + k] : 0.0;
         ft += v * v;
         T[e] = (D[i] > 0.0) ? v / D[i] : 0.0;
     }
-    for (int o = 32; o > 0; o >>= 1) ft += __shfl_xor(ft, o);
+    ft = block_sum(ft);
+    // lambda_max(G^-1) = ||Rs^-1||_2^2 by power iteration on Ts Ts^T
+    for (int i = tid; i < c; i += NT) pv[i] = 1.0 + 0.01 * (i % 5);
     __syncthreads();
-    if ((tid & 63) == 0) s_red[tid >> 6] = ft;
-    __syncthreads();
-    if (tid == 0) {
-        double t = 0.0;
-        for (int i = 0; i < NT / 64; i++) t += s_red[i];
-        dinfo[0] *= sqrt(t);
-        info[0] = s_def[0];
+    double lminv = 0.0;
+    for (int it = 0; it < PIT; it++) {
+        const int row = tid >> 3, sub = tid & 7;
+        // u = Ts^T x  (u_k = sum_{i<=k} Ts(i,k) x_i)
+        for (int k0 = 0; k0 < c; k0 += NT / 8) {
+            int k = k0 + row;
+            double acc = 0.0;
+            if (k < c)
+                for (int i = sub; i <= k; i += 8) acc += S[(long)i * c + k] * pv[i];
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 2);
+            acc += __shfl_xor(acc, 4);
+            if (k < c && sub == 0) col[k] = acc;
+        }
+        __syncthreads();
+        // y = Ts u  (y_i = sum_{k>=i} Ts(i,k) u_k)
+        for (int i0 = 0; i0 < c; i0 += NT / 8) {
+            int i = i0 + row;
+            double acc = 0.0;
+            if (i < c)
+                for (int k = i + sub; k < c; k += 8) acc += S[(long)i * c + k] * col[k];
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 2);
+            acc += __shfl_xor(acc, 4);
+            if (i < c && sub == 0) pv[c + i] = acc;
+        }
+        __syncthreads();
+        double nn = 0.0;
+        for (int i = tid; i < c; i += NT) nn += pv[c + i] * pv[c + i];
+        nn = block_sum(nn);
+        double xn = 0.0;
+        for (int i = tid; i < c; i += NT) xn += pv[i] * pv[i];
+        xn = block_sum(xn);
+        lminv = (xn > 0.0) ? sqrt(nn / xn) : 0.0;
+        double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+        for (int i = tid; i < c; i += NT) pv[i] = pv[c + i] * inv;
+        __syncthreads();
     }
+    if (tid == 0) {
+        dinfo[0] = sqrt(fr) * sqrt(ft);      // Frobenius condition (upper bound, >= c)
+        dinfo[1] = sqrt(lmax * lminv);       // 2-norm condition estimate of Rs
+        info[0] = s_def;
+    }
+    (void)s_scal;
 }
 
 __device__ __forceinline__ void rr_pair(int kp, int t, int p, int &a, int &b) {
@@ -147,16 +224,22 @@ __device__ __forceinline__ void rr_pair(int kp, int t, int p, int &a, int &b) {
     }
 }
 
+// INLDS: columns of A (and V when VLDS) live in LDS.  LOGROT: V is not
+// kept here; every pair's rotation (c, s) is logged and replayed afterwards
+// on the rows of V by k_svd_replay_v (rows of V evolve independently).
+template <bool INLDS, bool VLDS, bool LOGROT>
 __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *__restrict__ A,
                                                    long lda, double *__restrict__ sig,
                                                    double *__restrict__ U, long ldu,
                                                    double *__restrict__ V, long ldv,
                                                    double log_scale, double delta, int cap,
-                                                   int *iout, double *gwork, int use_lds) {
+                                                   int *iout, double *gwork,
+                                                   double2 *__restrict__ rotlog,
+                                                   int *__restrict__ posout) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
-    double *W = use_lds ? lds : gwork;            // m x k, ld m
-    double *Vw = V ? (use_lds ? lds + (long)m * k : gwork + (long)m * k) : nullptr;  // k x k
+    double *W = INLDS ? lds : gwork;  // m x k, ld m
+    double *Vw = (V && !LOGROT) ? (VLDS ? lds + (long)m * k : gwork + (long)m * k) : nullptr;
     __shared__ int s_rot;
     __shared__ double s_sig[1024];
     __shared__ double s_sorted[1024];
@@ -179,7 +262,10 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     while (G > 1 && G * P > NT) G >>= 1;
     const int ngroups = NT / G;
     const int g = tid / G, gl = tid - g * G;
-    const double tol = 0x1.0p-52 * sqrt((double)m);
+    // Rotate only when the pair's cosine exceeds the rounding noise of an
+    // m-term dot product; below that the columns are orthogonal to working
+    // accuracy (stricter thresholds never settle).
+    const double tol = 0x1.0p-53 * (double)m;
     int sweep = 0;
     const int max_sweeps = 40;
     for (; sweep < max_sweeps && kp >= 2; sweep++) {
@@ -207,7 +293,18 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
                     be += __shfl_xor(be, o);
                     ga += __shfl_xor(ga, o);
                 }
-                if (valid && fabs(ga) > tol * sqrt(al * be) && al > 0.0 && be > 0.0) {
+                bool rot = valid && fabs(ga) > tol * sqrt(al * be) && al > 0.0 && be > 0.0;
+                if (LOGROT && p < P && gl == 0) {
+                    double2 cslog = make_double2(1.0, 0.0);
+                    if (rot) {
+                        double zeta = (be - al) / (2.0 * ga);
+                        double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        double cs = 1.0 / sqrt(1.0 + tt * tt);
+                        cslog = make_double2(cs, cs * tt);
+                    }
+                    rotlog[((long)sweep * (kp - 1) + t) * P + p] = cslog;
+                }
+                if (rot) {
                     double zeta = (be - al) / (2.0 * ga);
                     double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                     double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
@@ -258,6 +355,7 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     for (int j = tid; j < k; j += NT) {
         sig[s_pos[j]] = s_sig[j];
         s_sorted[s_pos[j]] = s_sig[j];
+        if (LOGROT) posout[j] = s_pos[j];
     }
     for (long e = tid; e < (long)m * k; e += NT) {
         long j = e / m, i = e - j * m;
@@ -290,8 +388,41 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
         if (rho < 1) rho = 1;
         iout[0] = numrank;
         iout[1] = rho;
-        iout[2] = sweep + 1;
+        iout[2] = (sweep < max_sweeps) ? sweep + 1 : max_sweeps;
+        // sweeps written to the log: the loop breaks after a rotation-free
+        // sweep (logged as identities) or stops at max_sweeps
+        if (LOGROT) iout[3] = (sweep < max_sweeps) ? sweep + 1 : max_sweeps;
     }
+}
+
+// Replays the logged Jacobi rotations on the rows of V (V starts as I):
+// one wave per row, the row kept in LDS; each round applies the P disjoint
+// pair rotations in parallel.  Row i of V ends at V[i, pos[j]] = row[j].
+__global__ __launch_bounds__(64) void k_svd_replay_v(int k, const double2 *__restrict__ rotlog,
+                                                    const int *__restrict__ nsw,
+                                                    const int *__restrict__ pos,
+                                                    double *__restrict__ V, long ldv) {
+    __shared__ double row[1024];
+    const int i = blockIdx.x, lane = threadIdx.x;
+    const int kp = k + (k & 1), P = kp / 2;
+    for (int j = lane; j < kp; j += 64) row[j] = (j == i) ? 1.0 : 0.0;
+    __syncthreads();
+    const int sweeps = *nsw;
+    for (int sw = 0; sw < sweeps; sw++)
+        for (int t = 0; t < kp - 1; t++) {
+            for (int p = lane; p < P; p += 64) {
+                int a, b;
+                rr_pair(kp, t, p, a, b);
+                double2 cs = rotlog[((long)sw * (kp - 1) + t) * P + p];
+                if (a < k && b < k && cs.y != 0.0) {
+                    double x = row[a], y = row[b];
+                    row[a] = cs.x * x - cs.y * y;
+                    row[b] = cs.y * x + cs.x * y;
+                }
+            }
+            __syncthreads();
+        }
+    for (int j = lane; j < k; j += 64) V[i + (long)pos[j] * ldv] = row[j];
 }
 
 __global__ void k_svd_factors(int m, int rhomax, const int *__restrict__ drho,
@@ -310,18 +441,22 @@ __global__ void k_svd_factors(int m, int rhomax, const int *__restrict__ drho,
     }
 }
 
-}  // namespace
+template <typename KF>
+void set_lds_attr(KF kern) {
+    HIPCHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 LDS_DOUBLES * (int)sizeof(double)));
+}
 
 void allow_big_lds() {
     static bool done = false;
     if (done) return;
-    const int bytes = LDS_DOUBLES * (int)sizeof(double);
-    HIPCHECK(hipFuncSetAttribute((const void *)k_chol_qr_factor,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    HIPCHECK(hipFuncSetAttribute((const void *)k_svd_jacobi,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    set_lds_attr(k_chol_qr_factor<true>);
+    set_lds_attr(k_svd_jacobi<true, true, false>);
+    set_lds_attr(k_svd_jacobi<true, false, true>);
     done = true;
 }
+
+}  // namespace
 
 extern "C" {
 
@@ -329,16 +464,15 @@ void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny, d
                             double *T, int *info, double *dinfo) {
     ptt_hip_init();
     allow_big_lds();
-    size_t need = (size_t)c * c + 3 * (size_t)c;
-    double *gwork = nullptr;
-    size_t shmem = 0;
+    size_t need = (size_t)c * c + 5 * (size_t)c;
     if (need <= (size_t)LDS_DOUBLES) {
-        shmem = need * sizeof(double);
+        PTT_LAUNCH("chol_qr_factor", k_chol_qr_factor<true>, dim3(1), dim3(NT),
+                   need * sizeof(double), c, G, shift, tiny, R, T, info, dinfo, (double *)nullptr);
     } else {
-        gwork = (double *)ptt_hip_scratch(PTT_WS_SKETCH, need * sizeof(double));
+        double *gwork = (double *)ptt_hip_scratch(PTT_WS_SKETCH, need * sizeof(double));
+        PTT_LAUNCH("chol_qr_factor", k_chol_qr_factor<false>, dim3(1), dim3(NT), 0, c, G, shift,
+                   tiny, R, T, info, dinfo, gwork);
     }
-    PTT_LAUNCH("chol_qr_factor", k_chol_qr_factor, dim3(1), dim3(NT), shmem, c, G, shift, tiny, R,
-               T, info, dinfo, gwork);
 }
 
 void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *U, long ldu,
@@ -350,16 +484,31 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *
         fprintf(stderr, "paratt_amd: Jacobi SVD limited to k <= 1024 (k = %d)\n", k);
         exit(1);
     }
-    size_t need = (size_t)m * k + (V ? (size_t)k * k : 0);
-    int use_lds = need <= (size_t)LDS_DOUBLES ? 1 : 0;
-    double *gwork = nullptr;
-    size_t shmem = 0;
-    if (use_lds)
-        shmem = need * sizeof(double);
-    else
-        gwork = (double *)ptt_hip_scratch(PTT_WS_SKETCH, need * sizeof(double));
-    PTT_LAUNCH("svd_jacobi", k_svd_jacobi, dim3(1), dim3(NT), shmem, m, k, A, lda, sig, U, ldu, V,
-               ldv, log_scale, delta, cap, iout, gwork, use_lds);
+    const size_t mk = (size_t)m * k, kk = (size_t)k * k;
+    if (mk + (V ? kk : 0) <= (size_t)LDS_DOUBLES) {
+        PTT_LAUNCH("svd_jacobi", (k_svd_jacobi<true, true, false>), dim3(1), dim3(NT),
+                   (mk + (V ? kk : 0)) * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv,
+                   log_scale, delta, cap, iout, (double *)nullptr, (double2 *)nullptr,
+                   (int *)nullptr);
+    } else if (mk <= (size_t)LDS_DOUBLES) {
+        // columns in LDS; V (if wanted) from the rotation log
+        const int kp = k + (k & 1), P = kp / 2;
+        const size_t logn = (size_t)40 * (kp - 1) * P;
+        char *ws = (char *)ptt_hip_scratch(PTT_WS_SKETCH, logn * sizeof(double2) + 4096 * sizeof(int));
+        double2 *rotlog = (double2 *)ws;
+        int *pos = (int *)(ws + logn * sizeof(double2));
+        PTT_LAUNCH("svd_jacobi", (k_svd_jacobi<true, false, true>), dim3(1), dim3(NT),
+                   mk * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv, log_scale, delta, cap,
+                   iout, (double *)nullptr, rotlog, pos);
+        if (V)
+            PTT_LAUNCH("svd_replay_v", k_svd_replay_v, dim3(k), dim3(64), 0, k, rotlog, iout + 3,
+                       pos, V, ldv);
+    } else {
+        double *gwork = (double *)ptt_hip_scratch(PTT_WS_SKETCH, (mk + (V ? kk : 0)) * sizeof(double));
+        PTT_LAUNCH("svd_jacobi", (k_svd_jacobi<false, false, false>), dim3(1), dim3(NT), 0, m, k,
+                   A, lda, sig, U, ldu, V, ldv, log_scale, delta, cap, iout, gwork,
+                   (double2 *)nullptr, (int *)nullptr);
+    }
 }
 
 void ptt_hip_svd_factors(int m, int rhomax, const int *drho, const double *U, long ldu,

@@ -249,30 +249,58 @@ __global__ void k_transpose(int m, int n, const double *__restrict__ A, long lda
     }
 }
 
+__global__ void k_block_place(double *__restrict__ dst, int dr0, int n, int ra, int rb,
+                              const double *__restrict__ src, int sr0, int sr1, double alpha) {
+    long rows = (long)sr0 * n, total = rows * sr1;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        long b = e / rows;
+        long m = e - b * rows;
+        int j = (int)(m / sr0);
+        int a = (int)(m - (long)j * sr0);
+        dst[(ra + a) + (long)dr0 * (j + (long)n * (rb + b))] = alpha * src[e];
+    }
+}
+
+__global__ void k_fill(double *p, long count, double v) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < count;
+         e += (long)gridDim.x * blockDim.x)
+        p[e] = v;
+}
+
 __global__ void k_set_diag(double *A, int m, long lda, double alpha) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
         A[i + (long)i * lda] = alpha;
 }
 
-__global__ void k_sumsq(int m, int n, const double *__restrict__ A, long lda,
-                        double *out) {
-    __shared__ double red[1024 / 64];
+// Two-stage deterministic sum of squares: stage 1 gives one partial per
+// workgroup (grid-stride over the block), stage 2 adds the partials in order.
+__global__ __launch_bounds__(256) void k_sumsq_partial(int m, int n, const double *__restrict__ A,
+                                                       long lda, double *__restrict__ part) {
+    __shared__ double red[4];
     double s = 0.0;
     long total = (long)m * n;
-    for (long e = threadIdx.x; e < total; e += blockDim.x) {
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
         long j = e / m, i = e - j * m;
         double v = A[i + j * lda];
         s += v * v;
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) red[w] = s;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += red[i];
-        *out = t;
-    }
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void k_sum_final(int np, const double *__restrict__ part,
+                                                   double *out) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 uint64_t mix64_host(uint64_t z) {
@@ -379,6 +407,25 @@ void ptt_hip_transpose(int m, int n, const double *A, long lda, double *B, long 
                lda, B, ldb, alpha);
 }
 
+void ptt_hip_block_place(double *dst, int dr0, int n, int dr1, int ra, int rb, const double *src,
+                         int sr0, int sr1, double alpha) {
+    ptt_hip_init();
+    long total = (long)sr0 * n * sr1;
+    if (total <= 0) return;
+    if (ra + sr0 > dr0 || rb + sr1 > dr1) {
+        fprintf(stderr, "paratt_amd: block_place out of range\n");
+        exit(1);
+    }
+    PTT_LAUNCH_FB("block_place", 0.0, 16.0 * total, k_block_place, dim3(stream_blocks(total, 256)),
+                  dim3(256), 0, dst, dr0, n, ra, rb, src, sr0, sr1, alpha);
+}
+
+void ptt_hip_fill(double *p, long count, double v) {
+    ptt_hip_init();
+    if (count <= 0) return;
+    PTT_LAUNCH("fill", k_fill, dim3(stream_blocks(count, 256)), dim3(256), 0, p, count, v);
+}
+
 void ptt_hip_set_diag(double *A, int m, long lda, double alpha) {
     ptt_hip_init();
     if (m <= 0) return;
@@ -387,7 +434,14 @@ void ptt_hip_set_diag(double *A, int m, long lda, double alpha) {
 
 void ptt_hip_sumsq(int m, int n, const double *A, long lda, double *out) {
     ptt_hip_init();
-    PTT_LAUNCH("sumsq", k_sumsq, dim3(1), dim3(1024), 0, m, n, A, lda, out);
+    long total = (long)m * n;
+    int nb = (int)stream_blocks(total, 256);
+    if (nb > 1024) nb = 1024;
+    static double *g_part = nullptr;
+    if (!g_part) g_part = (double *)ptt_hip_malloc(sizeof(double) * 1024);
+    PTT_LAUNCH_FB("sumsq", 2.0 * total, 8.0 * total, k_sumsq_partial, dim3(nb), dim3(256), 0, m, n,
+                  A, lda, g_part);
+    PTT_LAUNCH("sum_final", k_sum_final, dim3(1), dim3(256), 0, nb, g_part, out);
 }
 
 }  // extern "C"

@@ -145,17 +145,40 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, int kchunk,
     }
 }
 
-__global__ void k_splitk_reduce(int M, int N, int nsplit, const double *__restrict__ partial,
-                                double alpha, double beta, double *C, long ldc, int sym) {
-    long total = (long)M * N;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-         e += (long)gridDim.x * blockDim.x) {
+// Sum of nsplit partial tiles.  A workgroup owns 64 output elements; its 4
+// waves take every 4th partial with 4 independent accumulators each, then
+// combine through LDS, so many loads are in flight per lane.
+__global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int nsplit,
+                                                       const double *__restrict__ partial,
+                                                       double alpha, double beta, double *C,
+                                                       long ldc, int sym) {
+    __shared__ double red[4][64];
+    const long total = (long)M * N;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long e = (long)blockIdx.x * 64 + lane;
+    double s = 0.0;
+    long src = 0;
+    if (e < total) {
         int n = (int)(e / M), m = (int)(e - (long)n * M);
-        long src = (sym && m > n) ? (long)n + (long)m * M : e;
-        double s = 0.0;
-        for (int z = 0; z < nsplit; z++) s += partial[(long)z * total + src];
+        src = (sym && m > n) ? (long)n + (long)m * M : e;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int z = wv;
+        for (; z + 12 < nsplit; z += 16) {
+            a0 += partial[(long)z * total + src];
+            a1 += partial[(long)(z + 4) * total + src];
+            a2 += partial[(long)(z + 8) * total + src];
+            a3 += partial[(long)(z + 12) * total + src];
+        }
+        for (; z < nsplit; z += 4) a0 += partial[(long)z * total + src];
+        s = (a0 + a1) + (a2 + a3);
+    }
+    red[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && e < total) {
+        double t = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        int n = (int)(e / M), m = (int)(e - (long)n * M);
         double *cp = C + m + (long)n * ldc;
-        *cp = (beta == 0.0) ? alpha * s : alpha * s + beta * (*cp);
+        *cp = (beta == 0.0) ? alpha * t : alpha * t + beta * (*cp);
     }
 }
 
@@ -221,9 +244,9 @@ void gemm_impl(char ta, char tb, int M, int N, int K, double alpha, const double
     int nsplit = 1;
     long tiles = (long)gm * gn;
     if ((force_split || tiles < 64) && K >= 1024) {
-        // aim at ~2 waves of workgroups over the 256 CUs, >= 256 of K each
-        long want = (512 + tiles - 1) / tiles;
-        long maxs = K / 256;
+        // one workgroup per CU (256) and >= 512 of K per split
+        long want = (256 + tiles - 1) / tiles;
+        long maxs = K / 512;
         nsplit = (int)(want < maxs ? want : maxs);
         if (nsplit < 1) nsplit = 1;
         if (nsplit > 1024) nsplit = 1024;
@@ -250,7 +273,7 @@ void gemm_impl(char ta, char tb, int M, int N, int K, double alpha, const double
              : launch_bt<128, 128, false>(ta, tb, grid, M, N, K, kchunk, alpha, A, lda, B, ldb, beta, C, ldc, partial, name);
     if (nsplit > 1) {
         long total = (long)M * N;
-        PTT_LAUNCH("splitk_reduce", k_splitk_reduce, dim3(stream_blocks(total, 256)), dim3(256), 0,
+        PTT_LAUNCH("splitk_reduce", k_splitk_reduce, dim3(cdiv(total, 64)), dim3(256), 0,
                    M, N, nsplit, partial, alpha, beta, C, ldc, syrk ? 1 : 0);
     }
 }

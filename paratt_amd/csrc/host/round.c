@@ -86,11 +86,13 @@ static chol_out run_chol(int c, const double *G, double shift, double *R, double
     chol_out o;
     ptt_hip_chol_qr_factor(c, G, shift, shift > 0 ? 0.0 : tiny_pivot(), R, T, w->iinfo, w->dinfo);
     int info = 0;
-    double kap = 0.0;
+    double kap[2] = {0.0, 0.0};
     ptt_hip_d2h(&info, w->iinfo, sizeof(int));
-    ptt_hip_d2h(&kap, w->dinfo, sizeof(double));
+    ptt_hip_d2h(kap, w->dinfo, 2 * sizeof(double));
     o.deficient = info;
-    o.kappa = kap;
+    /* 2-norm estimate of the scaled factor (power iteration converges from
+     * below; ||.||_F-condition / c is a guaranteed lower bound too) */
+    o.kappa = kap[1] > kap[0] / c ? kap[1] : kap[0] / c;
     return o;
 }
 
@@ -129,6 +131,8 @@ static int cholqr_core(attac_tt *x, int core, char trans, long len, int c, doubl
     double *A = x->coreList[core];
     ptt_hip_gram(trans, c, len, A, trans == 'T' ? len : c, w->G);
     chol_out o = run_chol(c, w->G, 0.0, w->R1, w->T1, w);
+    if (trans == 'T' && core < 64) g_stats.kappa_ltr[core] = o.kappa;
+    if (trans == 'N' && core < 64) g_stats.kappa_rtl[core] = o.kappa;
     if (one_pass) *one_pass = 0;
     if (o.deficient == 0 && o.kappa <= KAPPA_ONE_PASS) {
         if (allow_one_pass_noq) {
@@ -222,6 +226,116 @@ static void ltr_sweep(attac_tt *x, const double *qscale, const double *rscale) {
     }
 }
 
+/* Reference tail rule (pattac.c:468-481) on a descending list: the loop
+ * accumulates sig_j^2 from the bottom and lowers rho while sqrt(sum) < delta.
+ * tail0 is the energy beyond the list (0 for a full spectrum). */
+static int tail_rule(const double *sig, int nsig, double tail0, double delta) {
+    int rho = nsig;
+    double sum = tail0;
+    for (int j = nsig - 1; j >= 0; j--) {
+        sum += sig[j] * sig[j];
+        if (sqrt(sum) < delta) rho = j;
+        else break;
+    }
+    return rho;
+}
+
+/* Robust right-to-left step: explicit LQ by (shifted) Cholesky-QR, SVD of
+ * L with right vectors, new core = V_rho^T Q. */
+static int rtl_step_robust(attac_tt *x, int i, const int *maxr, double delta,
+                           const double *nscale, step_ws *ws, int *passes_out) {
+    int *r = x->ttRank, *n = x->dim;
+    int rows = r[i];
+    long w = (long)n[i] * r[i + 1];
+    const double *R = NULL, *T1 = NULL;
+    int one_pass = 0;
+    int passes = cholqr_core(x, i, 'N', w, rows, 1.0, ws, &R, 0, &one_pass, &T1);
+    *passes_out = passes;
+    ptt_hip_transpose(rows, rows, R, rows, ws->L, rows, 1.0);
+    ptt_hip_svd_jacobi(rows, rows, ws->L, rows, ws->sig, ws->U, rows, ws->V, rows, 0.0, delta,
+                       maxr[i], ws->iinfo + 1);
+    int io[3];
+    ptt_hip_d2h(io, ws->iinfo + 1, sizeof(int) * 3);
+    int rho = io[1];
+    if (i < 64) { g_stats.svd_sweeps[i] = io[2]; g_stats.numrank[i] = io[0]; }
+    double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)rho * w);
+    ptt_hip_svd_factors(rows, rho, ws->iinfo + 2, ws->U, rows, ws->sig, 1.0, NULL, ws->US);
+    ptt_hip_dgemm('T', 'N', rho, (int)w, rows, nscale[i], ws->V, rows, x->coreList[i], rows, 0.0,
+                  tmp, rho);
+    ptt_host_swap_core_with_tmp(x, i, (long)rho * w);
+    return rho;
+}
+
+/* Fast right-to-left step for a core whose kept directions are well
+ * conditioned: Gram -> guarded Cholesky -> Jacobi SVD of L (no V) ->
+ * new core = Sigma^-1 U^T M in one GEMM.  The discarded energy is measured
+ * from the explicit residual M - U Sigma (new core), accurate to rounding of
+ * M itself, so the tail rule is decided exactly as with an orthogonal LQ.
+ * Returns rho, or -1 when the step must be redone by the robust path (the
+ * core is left unchanged in that case). */
+static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
+                         const double *nscale, step_ws *ws) {
+    int *r = x->ttRank, *n = x->dim;
+    int rows = r[i];
+    long w = (long)n[i] * r[i + 1];
+    double *M = x->coreList[i];
+    ptt_hip_gram('N', rows, w, M, rows, ws->G);
+    chol_out o = run_chol(rows, ws->G, 0.0, ws->R1, ws->T1, ws);
+    if (i < 64) g_stats.kappa_rtl[i] = o.kappa;
+    ptt_hip_transpose(rows, rows, ws->R1, rows, ws->L, rows, 1.0);
+    ptt_hip_svd_jacobi(rows, rows, ws->L, rows, ws->sig, ws->U, rows, NULL, rows, 0.0, delta,
+                       maxr[i], ws->iinfo + 1);
+    int io[3];
+    ptt_hip_d2h(io, ws->iinfo + 1, sizeof(int) * 3);
+    double *sig = (double *)malloc(sizeof(double) * rows);
+    ptt_hip_d2h(sig, ws->sig, sizeof(double) * rows);
+    int numrank = io[0];
+    if (i < 64) { g_stats.svd_sweeps[i] = io[2]; g_stats.numrank[i] = numrank; }
+    int cap = maxr[i] < rows ? maxr[i] : rows;
+    int rho0 = cap < numrank ? cap : numrank;
+    if (rho0 < 1) rho0 = 1;
+    const double s0 = sig[0];
+    /* kept directions must be well conditioned: orthonormality of the new
+     * rows is ~ u * (s0 / s_{rho0-1})^2 */
+    if (!(s0 > 0.0) || sig[rho0 - 1] < 1e-2 * s0) { free(sig); return -1; }
+    int h_rho0 = rho0;
+    ptt_hip_h2d(ws->iinfo + 8, &h_rho0, sizeof(int));
+    double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)rho0 * w);
+    ptt_hip_svd_factors(rows, rho0, ws->iinfo + 8, ws->U, rows, ws->sig, nscale[i], ws->W, ws->US);
+    ptt_hip_dgemm('N', 'N', rho0, (int)w, rows, 1.0, ws->W, rho0, M, rows, 0.0, tmp, rho0);
+    /* residual in place: M <- M - U Sigma (new core)/sqrt(n_i); tail^2 = ||M||^2 */
+    ptt_hip_dgemm('N', 'N', rows, (int)w, rho0, -1.0 / nscale[i], ws->US, rows, tmp, rho0, 1.0, M,
+                  rows);
+    double *dss = ws->dinfo + 4;
+    ptt_hip_sumsq(rows, (int)w, M, rows, dss);
+    double tail2 = 0.0;
+    ptt_hip_d2h(&tail2, dss, sizeof(double));
+    int rho = rho0;
+    if (!(sqrt(tail2) >= delta)) {
+        /* the cut moves inside the kept list: trustworthy only when delta is
+         * far above the Gram's rounding floor */
+        const double floor2 = 1e4 * rows * 0x1.0p-53 * s0 * s0;
+        if (delta * delta < floor2) {
+            /* undo: M was overwritten by the residual, restore it */
+            ptt_hip_dgemm('N', 'N', rows, (int)w, rho0, 1.0 / nscale[i], ws->US, rows, tmp, rho0,
+                          1.0, M, rows);
+            free(sig);
+            return -2;
+        }
+        rho = tail_rule(sig, rho0, tail2, delta);
+        if (rho < 1) rho = 1;
+    }
+    if (rho < rho0) {
+        /* keep the first rho rows of the new core and the first rho columns of U Sigma */
+        double *t2 = (double *)ptt_hip_scratch(PTT_WS_TMP2, sizeof(double) * (size_t)rho * w);
+        ptt_hip_copy2d(rho, (int)w, tmp, rho0, t2, rho, 1.0);
+        ptt_hip_d2d(tmp, t2, sizeof(double) * (size_t)rho * w);
+    }
+    ptt_host_swap_core_with_tmp(x, i, (long)rho * w);
+    free(sig);
+    return rho;
+}
+
 /* Right-to-left truncation (pattac.c:297-522) of a left-orthogonal TT whose
  * cores carry the reference's scaled magnitudes; tol is absolute in that
  * scaled space.  nscale[k] = sqrt(n_k) is applied to each finished core. */
@@ -252,32 +366,13 @@ static void rtl_sweep(attac_tt *x, const int *maxr, double abs_tol, const double
             ptt_hip_svd_factors(rows, rho, ws.iinfo + 2, ws.U, rows, ws.sig, 1.0, NULL, ws.US);
         } else {
             ws = get_ws(rows);
-            const double *R = NULL, *T1 = NULL;
-            int one_pass = 0;
-            int passes = cholqr_core(x, i, 'N', w, rows, 1.0, &ws, &R, 1, &one_pass, &T1);
-            if (i < 64) g_stats.rtl_cholqr_passes[i] = passes;
-            /* L = R^T (rows x rows, lower) */
-            ptt_hip_transpose(rows, rows, R, rows, ws.L, rows, 1.0);
-            ptt_hip_svd_jacobi(rows, rows, ws.L, rows, ws.sig, ws.U, rows, one_pass ? NULL : ws.V,
-                               rows, 0.0, delta, cap, ws.iinfo + 1);
-            int io[3];
-            ptt_hip_d2h(io, ws.iinfo + 1, sizeof(int) * 3);
-            rho = io[1];
-            if (i < 64) { g_stats.svd_sweeps[i] = io[2]; g_stats.numrank[i] = io[0]; }
-            double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)rho * w);
-            if (one_pass) {
-                /* new core = sqrt(n_i) Sigma^-1 U_rho^T M  (rho x w) */
-                ptt_hip_svd_factors(rows, rho, ws.iinfo + 2, ws.U, rows, ws.sig, nscale[i], ws.W,
-                                    ws.US);
-                ptt_hip_dgemm('N', 'N', rho, (int)w, rows, 1.0, ws.W, rho, x->coreList[i], rows,
-                              0.0, tmp, rho);
-            } else {
-                /* new core = sqrt(n_i) V_rho^T Q */
-                ptt_hip_svd_factors(rows, rho, ws.iinfo + 2, ws.U, rows, ws.sig, 1.0, NULL, ws.US);
-                ptt_hip_dgemm('T', 'N', rho, (int)w, rows, nscale[i], ws.V, rows, x->coreList[i],
-                              rows, 0.0, tmp, rho);
+            rho = rtl_step_fast(x, i, maxr, delta, nscale, &ws);
+            if (i < 64) g_stats.rtl_cholqr_passes[i] = 1;
+            if (rho < 0) {
+                int passes = 0;
+                rho = rtl_step_robust(x, i, maxr, delta, nscale, &ws, &passes);
+                if (i < 64) g_stats.rtl_cholqr_passes[i] = 10 + passes;
             }
-            ptt_host_swap_core_with_tmp(x, i, (long)rho * w);
         }
         if (i < 64) {
             double sv[4] = {0, 0, 0, 0};
