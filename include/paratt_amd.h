@@ -43,6 +43,28 @@ typedef struct {
 } paratt_amd_round_stats;
 void paratt_amd_last_round_stats(paratt_amd_round_stats *out);
 
+/* S = sum_i D_i u for a 3-point periodic stencil D applied along every mode
+ * (paratt.c:665-770 semantics), as an exact rank-2r TT:
+ *   core 0 [D u_0 | u_0], core k [[u_k, 0], [D u_k, u_k]], core d-1
+ *   [u_{d-1}; D u_{d-1}].  s is resized. */
+void paratt_amd_mode_sum(paratt *u, const double *sten3, paratt *s);
+
+/* Reaction-Burgers vector field u_t = -u sum_i d_i u + nu lap u + u(1-u)
+ * on a periodic grid of spacing h (central differences), every internal sum
+ * rounded to eps (relative) with rank cap.  Parameters are set once by
+ * paratt_amd_rxn_burgers_setup (static state, the paratt_ode.h convention
+ * for vector fields); the field has the paratt_ode.h vecField signature. */
+void paratt_amd_rxn_burgers_setup(double nu, double h, double eps, int cap);
+void paratt_amd_rxn_burgers_field(paratt *u, paratt *out, int nargs, ...);
+
+/* Step-truncation classical RK4: stages u+dt/2 k1, u+dt/2 k2, u+dt k3 and
+ * the combination k1+2k2+2k3+k4, then u+dt/6 (that), each rounded to eps
+ * (relative) with rank cap.  No counterpart in the reference (which declares
+ * a midpoint step without a body, paratt_ode.h:113-120). */
+void paratt_amd_rk4_st(paratt *f_old, paratt *f_new,
+                       void (*vecField)(paratt *, paratt *, int, ...), double dt,
+                       double eps, int cap);
+
 /* Block until all queued device work of the library is done. */
 void paratt_amd_sync(void);
 

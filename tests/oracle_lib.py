@@ -261,3 +261,58 @@ def scale(x, alpha):
     lib().o_scale(C.c_int(y.n[0]), C.c_int(y.r[1]), C.c_double(alpha),
                   y.cores[0].ctypes.data_as(C.POINTER(C.c_double)))
     return y
+
+
+# ---------------------------------------------------------------- ODE composites
+def _out_tt(n, rmax, d):
+    cores = [np.zeros(rmax * n[k] * rmax) for k in range(d)]
+    return cores
+
+
+def mode_sum(x, sten3):
+    d = x.d
+    rmax = 2 * max(x.r)
+    cores = _out_tt(x.n, rmax, d)
+    rout = np.zeros(d + 1, dtype=np.int32)
+    s, sp = _dp(sten3)
+    lib().o_mode_sum(C.c_int(d), _ip(x.n).ctypes.data_as(C.POINTER(C.c_int)),
+                     _ip(x.r).ctypes.data_as(C.POINTER(C.c_int)), _ptrs(x.cores), sp,
+                     rout.ctypes.data_as(C.POINTER(C.c_int)), _ptrs(cores))
+    r = [int(v) for v in rout]
+    return HostTT(x.n, r, [cores[k][: r[k] * x.n[k] * r[k + 1]] for k in range(d)])
+
+
+def rxn_burgers_rhs(x, nu, h, eps, cap):
+    d = x.d
+    cores = _out_tt(x.n, cap, d)
+    rout = np.zeros(d + 1, dtype=np.int32)
+    lib().o_rxn_burgers_rhs(C.c_int(d), _ip(x.n).ctypes.data_as(C.POINTER(C.c_int)),
+                            _ip(x.r).ctypes.data_as(C.POINTER(C.c_int)), _ptrs(x.cores),
+                            C.c_double(nu), C.c_double(h), C.c_double(eps), C.c_int(cap),
+                            rout.ctypes.data_as(C.POINTER(C.c_int)), _ptrs(cores))
+    r = [int(v) for v in rout]
+    return HostTT(x.n, r, [cores[k][: r[k] * x.n[k] * r[k + 1]] for k in range(d)])
+
+
+def rxn_burgers_rk4(x, nu, h, dt, eps, cap, steps):
+    d = x.d
+    cores = _out_tt(x.n, cap, d)
+    rout = np.zeros(d + 1, dtype=np.int32)
+    lib().o_rxn_burgers_rk4(C.c_int(d), _ip(x.n).ctypes.data_as(C.POINTER(C.c_int)),
+                            _ip(x.r).ctypes.data_as(C.POINTER(C.c_int)), _ptrs(x.cores),
+                            C.c_double(nu), C.c_double(h), C.c_double(dt), C.c_double(eps),
+                            C.c_int(cap), C.c_int(steps),
+                            rout.ctypes.data_as(C.POINTER(C.c_int)), _ptrs(cores))
+    r = [int(v) for v in rout]
+    return HostTT(x.n, r, [cores[k][: r[k] * x.n[k] * r[k + 1]] for k in range(d)])
+
+
+def gaussian_ic(d, n, centre=np.pi, width=0.5):
+    """Rank-1 TT of exp(-|x - centre|^2 / width) on the periodic grid
+    x_j = 2 pi j / n (config 4's initial condition; per-mode factors)."""
+    xs = 2 * np.pi * np.arange(n) / n
+    cores = []
+    cen = np.broadcast_to(np.asarray(centre, dtype=np.float64), (d,))
+    for k in range(d):
+        cores.append(np.exp(-(xs - cen[k]) ** 2 / width))
+    return HostTT([n] * d, [1] * (d + 1), cores)

@@ -1,0 +1,104 @@
+"""GPU parity of the ODE layer: pattac_exp_euler_st against the compiled
+reference's golden trajectory, and the reaction-Burgers RK4 composite
+against the CPU oracle's identical composition."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import paratt_amd as P
+from gpu_util import to_host
+
+pytestmark = pytest.mark.gpu
+
+PP = C.POINTER(P.Paratt)
+VECFIELD = C.CFUNCTYPE(None, PP, PP, C.c_int)
+
+
+def _setup_lib():
+    L = P.lib()
+    L.pattac_exp_euler_st.argtypes = [PP, PP, PP, VECFIELD, C.c_double, C.c_double, C.c_double]
+    L.paratt_amd_mode_sum.argtypes = [PP, C.POINTER(C.c_double), PP]
+    L.paratt_amd_rxn_burgers_setup.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
+    L.paratt_amd_rk4_st.argtypes = [PP, PP, C.c_void_p, C.c_double, C.c_double, C.c_int]
+    return L
+
+
+def _clone(L, p):
+    q = L.alloc_ptt(p.contents.numDim, p.contents.dim, p.contents.x.contents.ttRank)
+    L.copy_rank_change_tt(q.contents.x, p.contents.x)
+    return q
+
+
+def test_euler_heat_matches_reference_golden():
+    L = _setup_lib()
+    g = O.golden()
+    f0 = O.HostTT.from_json(g["euler_in"])
+    h2inv = 1.0 / (2 * np.pi / 16) ** 2
+    st = (C.c_double * 3)(h2inv, -2 * h2inv, h2inv)
+
+    def heat_rhs(f, out, nargs):
+        # same composition as tests/golden/gen_golden.c heat_rhs
+        d = f.contents.numDim
+        t = _clone(L, f)
+        L.pattac_apply_periodic_stencil(t, 0, st, 3)
+        L.copy_rank_change_tt(out.contents.x, t.contents.x)
+        L.free_ptt(t)
+        for i in range(1, d):
+            t = _clone(L, f)
+            L.pattac_apply_periodic_stencil(t, i, st, 3)
+            ro = out.contents.x.contents.ttRank
+            rt = t.contents.x.contents.ttRank
+            rz = (C.c_int * (d + 1))(*[1 if k in (0, d) else ro[k] + rt[k] for k in range(d + 1)])
+            acc = L.alloc_ptt(d, f.contents.dim, rz)
+            L.attac_tt_axpby(1.0, out.contents.x, 1.0, t.contents.x, acc.contents.x)
+            L.copy_rank_change_tt(out.contents.x, acc.contents.x)
+            L.paratt_truncate(out, 1e-12, out.contents.x.contents.ttRank)
+            L.free_ptt(acc)
+            L.free_ptt(t)
+
+    cb = VECFIELD(heat_rhs)
+    f = P.DevTT.from_host(f0)
+    fn = P.DevTT(f0.n, f0.r)
+    wk = P.DevTT(f0.n, f0.r)
+    for _ in range(3):
+        L.pattac_exp_euler_st(f.p, fn.p, wk.p, cb, 1e-3, 1e-10, 1e-12)
+        L.copy_rank_change_tt(f.tt, fn.tt)
+    out = to_host(f)
+    ref = O.HostTT.from_json(g["euler_out"])
+    assert out.r == ref.r
+    assert O.rel_diff(out, ref) < 1e-12
+
+
+def test_mode_sum_bit_exact_vs_oracle():
+    L = _setup_lib()
+    x = O.gauss_tt([9, 8, 10, 7], [1, 3, 4, 2, 1], 12)
+    st = np.array([-0.25, 0.1, 0.4])
+    ref = O.mode_sum(x, st)
+    dx = P.DevTT.from_host(x)
+    ds = P.DevTT(x.n, [1] * 5)
+    L.paratt_amd_mode_sum(dx.p, st.ctypes.data_as(C.POINTER(C.c_double)), ds.p)
+    assert ds.r == ref.r
+    for k, c in enumerate(ds.download_cores()):
+        np.testing.assert_array_equal(c, ref.cores[k])
+
+
+@pytest.mark.parametrize("d,n", [(4, 16), (6, 32)])
+def test_rxn_burgers_rk4_matches_oracle(d, n):
+    L = _setup_lib()
+    h = 2 * np.pi / n
+    nu, dt, eps, cap = 0.01, 1e-3, 1e-8, 64
+    u0 = O.gaussian_ic(d, n)
+    L.paratt_amd_rxn_burgers_setup(nu, h, eps, cap)
+    field = C.cast(L.paratt_amd_rxn_burgers_field, C.c_void_p)
+    u = P.DevTT.from_host(u0)
+    un = P.DevTT(u0.n, u0.r)
+    steps = 3
+    for _ in range(steps):
+        L.paratt_amd_rk4_st(u.p, un.p, field, dt, eps, cap)
+        L.copy_rank_change_tt(u.tt, un.tt)
+    out = to_host(u)
+    ref = O.rxn_burgers_rk4(u0, nu, h, dt, eps, cap, steps)
+    assert out.r == ref.r, (out.r, ref.r)
+    assert O.rel_diff(out, ref) < 1e-12 + 10 * eps
