@@ -66,11 +66,15 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-d", type=int, default=6)
+    ap.add_argument("--workload", choices=["round", "rk4"], default="round")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build targets one GPU; the TT lives on a single MI355X "
               "(multi-GPU sharding is a design note only). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "rk4":
+        return bench_rk4(a)
 
     cpu = None
     if not a.no_cpu:
@@ -195,6 +199,69 @@ def main():
         "cholqr_passes_rtl": sorted(set(stats.rtl_cholqr_passes[1:d])),
         "kappa_ltr_max": round(max(stats.kappa_ltr[:d - 1]), 3),
         "kappa_rtl_max": round(max(stats.kappa_rtl[1:d]), 3),
+    }
+    print(json.dumps(line))
+    sys.stdout.flush()
+
+
+def bench_rk4(a):
+    """Config 4: reaction-Burgers in TT, d=20, periodic n=256 on [0, 2pi),
+    nu=0.01, u_t = -u sum_i d_i u + nu lap u + u(1-u), Gaussian rank-1 IC
+    exp(-|x-pi|^2/0.5), step-truncation RK4 with dt=1e-3, every rounding at
+    relative eps 1e-8 with ranks capped at 64.  A step is one RK4 step; the
+    warmup steps advance the solution first (ranks grow from 1)."""
+    import ctypes as C
+    import math
+
+    import numpy as np
+
+    import paratt_amd as P
+
+    d, n = 20, 256
+    nu, dt, eps, cap = 0.01, 1e-3, 1e-8, 64
+    h = 2 * math.pi / n
+    L = P.lib()
+    PP = C.POINTER(P.Paratt)
+    L.paratt_amd_rxn_burgers_setup.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
+    L.paratt_amd_rk4_st.argtypes = [PP, PP, C.c_void_p, C.c_double, C.c_double, C.c_int]
+    L.paratt_amd_rxn_burgers_setup(nu, h, eps, cap)
+    field = C.cast(L.paratt_amd_rxn_burgers_field, C.c_void_p)
+    xs = 2 * np.pi * np.arange(n) / n
+    v = np.exp(-(xs - np.pi) ** 2 / 0.5)
+    u = P.DevTT([n] * d, [1] * (d + 1))
+    for k in range(d):
+        L.paratt_amd_core_upload(u.tt, k, v.ctypes.data_as(C.POINTER(C.c_double)))
+    un = P.DevTT([n] * d, [1] * (d + 1))
+    for _ in range(a.warmup):
+        L.paratt_amd_rk4_st(u.p, un.p, field, dt, eps, cap)
+        L.copy_rank_change_tt(u.tt, un.tt)
+    P.sync()
+    P.prof_reset()
+    P.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        L.paratt_amd_rk4_st(u.p, un.p, field, dt, eps, cap)
+        L.copy_rank_change_tt(u.tt, un.tt)
+    P.sync()
+    t1 = time.perf_counter()
+    P.prof_enable(False)
+    ms = (t1 - t0) * 1000.0 / a.steps
+    kern = []
+    for name in P.prof_names():
+        cnt, tms = P.prof_get(name)
+        kern.append({"name": name, "launches": cnt, "ms_per_step": round(tms / a.steps, 3)})
+    kern.sort(key=lambda k: -k["ms_per_step"])
+    busy = sum(k["ms_per_step"] for k in kern)
+    line = {
+        "metric": "reaction-Burgers RK4 step ms (d=20, n=256, eps 1e-8, cap 64)",
+        "value": round(ms, 3), "unit": "ms", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: Gaussian rank-1 initial condition",
+        "config": {"workload": "reaction-Burgers RK4, d=20, n=256, nu=0.01, dt=1e-3",
+                   "time_after_warmup": round(dt * a.warmup, 6)},
+        "ranks": u.r, "gpu_busy_ms_per_step": round(busy, 3),
+        "kernels": kern[:10],
     }
     print(json.dumps(line))
     sys.stdout.flush()

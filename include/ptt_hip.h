@@ -113,9 +113,12 @@ void ptt_hip_gram(char trans, int c, long len, const double *A, long lda,
  * `tiny` mark the index deficient (row of R and column of T set to 0).
  * Outputs (ld c): R = Rs*D (upper), T = D^-1 * Rs^-1 (upper, deficient
  * columns 0) so that A*T has orthonormal columns and A = (A*T)*R.
- * info[0] = number of deficient pivots, dinfo[0] = ||Rs||_F*||Rs^-1||_F. */
+ * info[0] = dinfo[2] = number of deficient pivots, dinfo[0] =
+ * ||Rs||_F*||Rs^-1||_F, dinfo[1] = 2-norm condition of Rs by power
+ * iteration when want_kappa (else 0). */
 void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny,
-                            double *R, double *T, int *info, double *dinfo);
+                            double *R, double *T, int *info, double *dinfo,
+                            int want_kappa);
 
 /* One-sided Jacobi SVD of A (m x k, lda, m >= k), FP64.
  * On return: sig[0..k) descending, U (m x k, ldu) left vectors,
@@ -123,7 +126,8 @@ void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny,
  * numrank (sig_j > sqrt(k)*eps*sig_0) and the truncation rank by the
  * reference's tail rule (pattac.c:468-481) applied to sig*exp(log_scale)
  * against delta, capped by min(k, cap), floor 1: written to iout[0]=numrank,
- * iout[1]=rho, iout[2]=sweeps.  A is overwritten. */
+ * iout[1]=rho, iout[2]=sweeps; iout needs 5 ints.  Exactly-zero columns of
+ * A are excluded from the rotations (their sigma is 0).  A is not changed. */
 void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig,
                         double *U, long ldu, double *V, long ldv,
                         double log_scale, double delta, int cap, int *iout);
@@ -133,6 +137,16 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig,
 void ptt_hip_svd_factors(int m, int rhomax, const int *drho, const double *U,
                          long ldu, const double *sig, double wscale, double *W,
                          double *US);
+/* ---- rank-predictor network (FP32, 18 -> 64 -> 64 -> 64 -> 1, tanh-GELU) ----
+ * X: nsamp x 18 row-major, Y: nsamp targets, P: ptt_hip_mlp_nparam() floats. */
+int ptt_hip_mlp_nparam(void);
+void ptt_hip_mlp_forward(int nsamp, const float *X, const float *P, float *out);
+/* grad (nparam floats) and loss (1 float) are overwritten: mean-squared
+ * error gradient over the batch. */
+void ptt_hip_mlp_grad(int nsamp, const float *X, const float *Y, const float *P,
+                      float *grad, float *loss);
+void ptt_hip_mlp_adam(float *P, const float *grad, float *m, float *v, float lr,
+                      int t);
 /* Small c x c triangular product C = A*B (upper times upper), ld c. */
 void ptt_hip_trmm_small(int c, const double *A, const double *B, double *C);
 

@@ -21,6 +21,9 @@ namespace {
 
 constexpr int NT = 1024;
 constexpr int LDS_DOUBLES = 17408;  // 136 KB dynamic + static arrays <= 160 KB LDS
+// The Jacobi kernel holds 24 KB of static LDS (sorting and packing tables),
+// so its dynamic budget is smaller: 17280 doubles + 24.1 KB < 160 KB.
+constexpr int SVD_LDS_DOUBLES = 17280;
 
 // Work matrix S is stored ROW-major in LDS (S[i*c + k] = entry (i,k)) so the
 // pivot row read by every thread in the trailing update is contiguous and
@@ -30,7 +33,8 @@ __global__ __launch_bounds__(NT) void k_chol_qr_factor(int c, const double *__re
                                                        double shift, double tiny,
                                                        double *__restrict__ R,
                                                        double *__restrict__ T, int *info,
-                                                       double *dinfo, double *gwork) {
+                                                       double *dinfo, double *gwork,
+                                                       int want_kappa) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
     // INLDS: the work matrix is LDS-resident (the compiler sees the shared
@@ -68,7 +72,7 @@ __global__ __launch_bounds__(NT) void k_chol_qr_factor(int c, const double *__re
         for (int i = 0; i < NT / 64; i++) t += s_red[i];
         return t;
     };
-    const int PIT = 12;
+    const int PIT = want_kappa ? 10 : 0;
     for (int i = tid; i < c; i += NT) pv[i] = 1.0 + 0.01 * (i % 7);
     __syncthreads();
     double lmax = 0.0;
@@ -206,7 +210,8 @@ __global__ __launch_bounds__(NT) void k_chol_qr_factor(int c, const double *__re
     }
     if (tid == 0) {
         dinfo[0] = sqrt(fr) * sqrt(ft);      // Frobenius condition (upper bound, >= c)
-        dinfo[1] = sqrt(lmax * lminv);       // 2-norm condition estimate of Rs
+        dinfo[1] = want_kappa ? sqrt(lmax * lminv) : 0.0;  // 2-norm condition of Rs
+        dinfo[2] = (double)s_def;
         info[0] = s_def;
     }
     (void)s_scal;
@@ -245,18 +250,40 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     __shared__ double s_sorted[1024];
     __shared__ int s_pos[1024];
 
+    // Exactly-zero columns (deficient directions) take no part in the
+    // rotations: nonzero columns are packed first, in their original order.
+    __shared__ int s_perm[1024];
+    __shared__ int s_knz;
+    for (int j = tid; j < k; j += NT) {
+        double s = 0.0;
+        for (int i = 0; i < m; i++) s = fmax(s, fabs(A[i + (long)j * lda]));
+        s_sig[j] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int q = 0;
+        for (int j = 0; j < k; j++)
+            if (s_sig[j] > 0.0) s_perm[q++] = j;
+        s_knz = q;
+        for (int j = 0; j < k; j++)
+            if (!(s_sig[j] > 0.0)) s_perm[q++] = j;
+    }
+    __syncthreads();
     for (long e = tid; e < (long)m * k; e += NT) {
         long j = e / m, i = e - j * m;
-        W[e] = A[i + j * lda];
+        W[e] = A[i + (long)s_perm[j] * lda];
     }
     if (Vw)
         for (long e = tid; e < (long)k * k; e += NT) {
             long j = e / k, i = e - j * k;
-            Vw[e] = (i == j) ? 1.0 : 0.0;
+            Vw[e] = (i == s_perm[j]) ? 1.0 : 0.0;
         }
+    if (LOGROT)
+        for (int j = tid; j < k; j += NT) posout[1024 + j] = s_perm[j];
     __syncthreads();
+    const int knz = s_knz;
 
-    const int kp = k + (k & 1);
+    const int kp = knz + (knz & 1);
     const int P = kp / 2;
     int G = 64;
     while (G > 1 && G * P > NT) G >>= 1;
@@ -268,7 +295,7 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     const double tol = 0x1.0p-53 * (double)m;
     int sweep = 0;
     const int max_sweeps = 40;
-    for (; sweep < max_sweeps && kp >= 2; sweep++) {
+    for (; sweep < max_sweeps && knz >= 2; sweep++) {
         if (tid == 0) s_rot = 0;
         __syncthreads();
         int rot_local = 0;
@@ -277,7 +304,7 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
                 int p = p0 + g;
                 int a = -1, b = -1;
                 if (p < P) rr_pair(kp, t, p, a, b);
-                bool valid = (p < P) && a < k && b < k;
+                bool valid = (p < P) && a < knz && b < knz;
                 double al = 0.0, be = 0.0, ga = 0.0;
                 if (valid) {
                     const double *ca = W + (long)a * m, *cb = W + (long)b * m;
@@ -392,6 +419,7 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
         // sweeps written to the log: the loop breaks after a rotation-free
         // sweep (logged as identities) or stops at max_sweeps
         if (LOGROT) iout[3] = (sweep < max_sweeps) ? sweep + 1 : max_sweeps;
+        if (LOGROT) iout[4] = knz;
     }
 }
 
@@ -404,8 +432,10 @@ __global__ __launch_bounds__(64) void k_svd_replay_v(int k, const double2 *__res
                                                     double *__restrict__ V, long ldv) {
     __shared__ double row[1024];
     const int i = blockIdx.x, lane = threadIdx.x;
-    const int kp = k + (k & 1), P = kp / 2;
-    for (int j = lane; j < kp; j += 64) row[j] = (j == i) ? 1.0 : 0.0;
+    const int knz = nsw[1];
+    const int kp = knz + (knz & 1), P = kp / 2;
+    const int *perm = pos + 1024;
+    for (int j = lane; j < 1024; j += 64) row[j] = (j < k && perm[j] == i) ? 1.0 : 0.0;
     __syncthreads();
     const int sweeps = *nsw;
     for (int sw = 0; sw < sweeps; sw++)
@@ -414,7 +444,7 @@ __global__ __launch_bounds__(64) void k_svd_replay_v(int k, const double2 *__res
                 int a, b;
                 rr_pair(kp, t, p, a, b);
                 double2 cs = rotlog[((long)sw * (kp - 1) + t) * P + p];
-                if (a < k && b < k && cs.y != 0.0) {
+                if (a < knz && b < knz && cs.y != 0.0) {
                     double x = row[a], y = row[b];
                     row[a] = cs.x * x - cs.y * y;
                     row[b] = cs.y * x + cs.x * y;
@@ -442,17 +472,17 @@ __global__ void k_svd_factors(int m, int rhomax, const int *__restrict__ drho,
 }
 
 template <typename KF>
-void set_lds_attr(KF kern) {
+void set_lds_attr(KF kern, int ndoubles) {
     HIPCHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 LDS_DOUBLES * (int)sizeof(double)));
+                                 ndoubles * (int)sizeof(double)));
 }
 
 void allow_big_lds() {
     static bool done = false;
     if (done) return;
-    set_lds_attr(k_chol_qr_factor<true>);
-    set_lds_attr(k_svd_jacobi<true, true, false>);
-    set_lds_attr(k_svd_jacobi<true, false, true>);
+    set_lds_attr(k_chol_qr_factor<true>, LDS_DOUBLES);
+    set_lds_attr(k_svd_jacobi<true, true, false>, SVD_LDS_DOUBLES);
+    set_lds_attr(k_svd_jacobi<true, false, true>, SVD_LDS_DOUBLES);
     done = true;
 }
 
@@ -461,17 +491,18 @@ void allow_big_lds() {
 extern "C" {
 
 void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny, double *R,
-                            double *T, int *info, double *dinfo) {
+                            double *T, int *info, double *dinfo, int want_kappa) {
     ptt_hip_init();
     allow_big_lds();
     size_t need = (size_t)c * c + 5 * (size_t)c;
     if (need <= (size_t)LDS_DOUBLES) {
         PTT_LAUNCH("chol_qr_factor", k_chol_qr_factor<true>, dim3(1), dim3(NT),
-                   need * sizeof(double), c, G, shift, tiny, R, T, info, dinfo, (double *)nullptr);
+                   need * sizeof(double), c, G, shift, tiny, R, T, info, dinfo, (double *)nullptr,
+                   want_kappa);
     } else {
         double *gwork = (double *)ptt_hip_scratch(PTT_WS_SKETCH, need * sizeof(double));
         PTT_LAUNCH("chol_qr_factor", k_chol_qr_factor<false>, dim3(1), dim3(NT), 0, c, G, shift,
-                   tiny, R, T, info, dinfo, gwork);
+                   tiny, R, T, info, dinfo, gwork, want_kappa);
     }
 }
 
@@ -485,16 +516,16 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *
         exit(1);
     }
     const size_t mk = (size_t)m * k, kk = (size_t)k * k;
-    if (mk + (V ? kk : 0) <= (size_t)LDS_DOUBLES) {
+    if (mk + (V ? kk : 0) <= (size_t)SVD_LDS_DOUBLES) {
         PTT_LAUNCH("svd_jacobi", (k_svd_jacobi<true, true, false>), dim3(1), dim3(NT),
                    (mk + (V ? kk : 0)) * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv,
                    log_scale, delta, cap, iout, (double *)nullptr, (double2 *)nullptr,
                    (int *)nullptr);
-    } else if (mk <= (size_t)LDS_DOUBLES) {
+    } else if (mk <= (size_t)SVD_LDS_DOUBLES) {
         // columns in LDS; V (if wanted) from the rotation log
         const int kp = k + (k & 1), P = kp / 2;
         const size_t logn = (size_t)40 * (kp - 1) * P;
-        char *ws = (char *)ptt_hip_scratch(PTT_WS_SKETCH, logn * sizeof(double2) + 4096 * sizeof(int));
+        char *ws = (char *)ptt_hip_scratch(PTT_WS_SKETCH, logn * sizeof(double2) + 2048 * sizeof(int));
         double2 *rotlog = (double2 *)ws;
         int *pos = (int *)(ws + logn * sizeof(double2));
         PTT_LAUNCH("svd_jacobi", (k_svd_jacobi<true, false, true>), dim3(1), dim3(NT),

@@ -47,7 +47,7 @@ typedef struct {
 static step_ws get_ws(int c) {
     step_ws w;
     size_t cc = (size_t)c * c;
-    double *base = (double *)ptt_hip_scratch(PTT_WS_SMALL, sizeof(double) * (15 * cc + 2 * (size_t)c + 64));
+    double *base = (double *)ptt_hip_scratch(PTT_WS_SMALL, sizeof(double) * (15 * cc + 2 * (size_t)c + 128));
     w.G = base;
     w.R1 = w.G + cc;
     w.T1 = w.R1 + cc;
@@ -63,8 +63,9 @@ static step_ws get_ws(int c) {
     w.W = w.V + cc;
     w.US = w.W + cc;
     w.sig = w.US + cc;
-    w.dinfo = w.sig + c + 8;
-    w.iinfo = (int *)ptt_hip_scratch(PTT_WS_INT, sizeof(int) * 64);
+    /* sig (c) | dinfo (8 doubles) | iinfo (32 ints): one d2h reads them all */
+    w.dinfo = w.sig + c;
+    w.iinfo = (int *)(w.dinfo + 8);
     w.cap = c;
     return w;
 }
@@ -82,18 +83,22 @@ typedef struct {
     double kappa;
 } chol_out;
 
-static chol_out run_chol(int c, const double *G, double shift, double *R, double *T, step_ws *w) {
+static chol_out run_chol_k(int c, const double *G, double shift, double *R, double *T,
+                           step_ws *w, int want_kappa) {
     chol_out o;
-    ptt_hip_chol_qr_factor(c, G, shift, shift > 0 ? 0.0 : tiny_pivot(), R, T, w->iinfo, w->dinfo);
-    int info = 0;
-    double kap[2] = {0.0, 0.0};
-    ptt_hip_d2h(&info, w->iinfo, sizeof(int));
-    ptt_hip_d2h(kap, w->dinfo, 2 * sizeof(double));
-    o.deficient = info;
+    ptt_hip_chol_qr_factor(c, G, shift, shift > 0 ? 0.0 : tiny_pivot(), R, T, w->iinfo, w->dinfo,
+                           want_kappa);
+    double kap[3] = {0.0, 0.0, 0.0};
+    ptt_hip_d2h(kap, w->dinfo, 3 * sizeof(double)); /* one transfer, waits for the kernel */
+    o.deficient = (int)kap[2];
     /* 2-norm estimate of the scaled factor (power iteration converges from
      * below; ||.||_F-condition / c is a guaranteed lower bound too) */
     o.kappa = kap[1] > kap[0] / c ? kap[1] : kap[0] / c;
     return o;
+}
+
+static chol_out run_chol(int c, const double *G, double shift, double *R, double *T, step_ws *w) {
+    return run_chol_k(c, G, shift, R, T, w, 1);
 }
 
 /* In-place C = alpha * A * T (A: len x c, ld lda; T upper c x c).  Uses the
@@ -280,15 +285,17 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
     long w = (long)n[i] * r[i + 1];
     double *M = x->coreList[i];
     ptt_hip_gram('N', rows, w, M, rows, ws->G);
-    chol_out o = run_chol(rows, ws->G, 0.0, ws->R1, ws->T1, ws);
+    chol_out o = run_chol_k(rows, ws->G, 0.0, ws->R1, ws->T1, ws, 0);
     if (i < 64) g_stats.kappa_rtl[i] = o.kappa;
     ptt_hip_transpose(rows, rows, ws->R1, rows, ws->L, rows, 1.0);
     ptt_hip_svd_jacobi(rows, rows, ws->L, rows, ws->sig, ws->U, rows, NULL, rows, 0.0, delta,
                        maxr[i], ws->iinfo + 1);
-    int io[3];
-    ptt_hip_d2h(io, ws->iinfo + 1, sizeof(int) * 3);
-    double *sig = (double *)malloc(sizeof(double) * rows);
-    ptt_hip_d2h(sig, ws->sig, sizeof(double) * rows);
+    /* sigma, factor info and SVD info are contiguous: one transfer */
+    size_t nb = sizeof(double) * ((size_t)rows + 8) + sizeof(int) * 8;
+    double *hb = (double *)malloc(nb);
+    ptt_hip_d2h(hb, ws->sig, nb);
+    double *sig = hb;
+    const int *io = (const int *)(hb + rows + 8) + 1;
     int numrank = io[0];
     if (i < 64) { g_stats.svd_sweeps[i] = io[2]; g_stats.numrank[i] = numrank; }
     int cap = maxr[i] < rows ? maxr[i] : rows;
@@ -297,7 +304,7 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
     const double s0 = sig[0];
     /* kept directions must be well conditioned: orthonormality of the new
      * rows is ~ u * (s0 / s_{rho0-1})^2 */
-    if (!(s0 > 0.0) || sig[rho0 - 1] < 1e-2 * s0) { free(sig); return -1; }
+    if (!(s0 > 0.0) || sig[rho0 - 1] < 1e-2 * s0) { free(hb); return -1; }
     int h_rho0 = rho0;
     ptt_hip_h2d(ws->iinfo + 8, &h_rho0, sizeof(int));
     double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)rho0 * w);
@@ -311,19 +318,28 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
     double tail2 = 0.0;
     ptt_hip_d2h(&tail2, dss, sizeof(double));
     int rho = rho0;
-    if (!(sqrt(tail2) >= delta)) {
+    int trust;
+    if (sqrt(tail2) >= delta) {
+        /* the tail beyond the kept list is too large: that is the answer only
+         * when the cap binds.  Otherwise the exact rounding keeps directions
+         * below the Gram's resolution, which this step cannot resolve. */
+        trust = (rho0 == cap);
+    } else {
         /* the cut moves inside the kept list: trustworthy only when delta is
          * far above the Gram's rounding floor */
         const double floor2 = 1e4 * rows * 0x1.0p-53 * s0 * s0;
-        if (delta * delta < floor2) {
-            /* undo: M was overwritten by the residual, restore it */
-            ptt_hip_dgemm('N', 'N', rows, (int)w, rho0, 1.0 / nscale[i], ws->US, rows, tmp, rho0,
-                          1.0, M, rows);
-            free(sig);
-            return -2;
+        trust = !(delta * delta < floor2);
+        if (trust) {
+            rho = tail_rule(sig, rho0, tail2, delta);
+            if (rho < 1) rho = 1;
         }
-        rho = tail_rule(sig, rho0, tail2, delta);
-        if (rho < 1) rho = 1;
+    }
+    if (!trust) {
+        /* undo: M was overwritten by the residual, restore it */
+        ptt_hip_dgemm('N', 'N', rows, (int)w, rho0, 1.0 / nscale[i], ws->US, rows, tmp, rho0, 1.0,
+                      M, rows);
+        free(hb);
+        return -2;
     }
     if (rho < rho0) {
         /* keep the first rho rows of the new core and the first rho columns of U Sigma */
@@ -332,7 +348,7 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
         ptt_hip_d2d(tmp, t2, sizeof(double) * (size_t)rho * w);
     }
     ptt_host_swap_core_with_tmp(x, i, (long)rho * w);
-    free(sig);
+    free(hb);
     return rho;
 }
 

@@ -7,8 +7,9 @@ import paratt_amd as P
 
 
 class DevBuf:
-    def __init__(self, arr):
-        arr = np.ascontiguousarray(arr, dtype=np.float64)
+    def __init__(self, arr, dtype=np.float64):
+        arr = np.ascontiguousarray(arr, dtype=dtype)
+        self.dtype = arr.dtype
         L = P.lib()
         L.ptt_hip_malloc.restype = C.c_void_p
         L.ptt_hip_malloc.argtypes = [C.c_size_t]
@@ -21,7 +22,7 @@ class DevBuf:
             L.ptt_hip_h2d(self.ptr, arr.ctypes.data, arr.nbytes)
 
     def get(self):
-        out = np.empty(self.n)
+        out = np.empty(self.n, dtype=self.dtype)
         if self.n:
             P.lib().ptt_hip_d2h(out.ctypes.data, self.ptr, out.nbytes)
         return out
