@@ -24,6 +24,7 @@ constexpr int LDS_DOUBLES = 17408;  // 136 KB dynamic + static arrays <= 160 KB 
 // The Jacobi kernel holds 24 KB of static LDS (sorting and packing tables),
 // so its dynamic budget is smaller: 17280 doubles + 24.1 KB < 160 KB.
 constexpr int SVD_LDS_DOUBLES = 17280;
+constexpr int SVD_RV = 8;  // rows per lane held in registers by a Jacobi pair
 
 // Work matrix S is stored ROW-major in LDS (S[i*c + k] = entry (i,k)) so the
 // pivot row read by every thread in the trailing update is contiguous and
@@ -217,6 +218,269 @@ __global__ __launch_bounds__(NT) void k_chol_qr_factor(int c, const double *__re
     (void)s_scal;
 }
 
+// Low-latency variant for c <= 128 (the whole problem in LDS): one barrier
+// per elimination step (the scaling of pivot row j is deferred to step j+1,
+// which does not read that row), and a blocked triangular inverse whose
+// strictly upper part is kept transposed in the unused lower triangle of S
+// (diagonal in tdg).  Same outputs and conventions as k_chol_qr_factor.
+constexpr int CHOL_FAST_C = 128;
+constexpr int CHOL_BLK = 32;
+constexpr int CHOL_LDS_DOUBLES = 20400;  // c^2 + 5c + 3*32*32 at c = 128; 159.4 KB
+
+__global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restrict__ G,
+                                                  double shift, double tiny,
+                                                  double *__restrict__ R, double *__restrict__ T,
+                                                  int *info, double *dinfo, int want_kappa) {
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    double *S = lds;              // c x c row-major
+    double *D = S + (long)c * c;  // Jacobi scaling
+    double *tdg = D + c;          // raw pivots, then the diagonal of Ts
+    double *dd = tdg + c;         // deficient flags
+    double *pv = dd + c;          // power-iteration vectors (2c)
+    double *Y = pv + 2 * c;       // 3 blocks of 32 x 32 (blocked inverse)
+    __shared__ int s_def;
+    __shared__ double s_red[2 * (NT / 64)];
+
+    auto block_sum2 = [&](double &a, double &b) {
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+        }
+        __syncthreads();
+        if ((tid & 63) == 0) {
+            s_red[2 * (tid >> 6)] = a;
+            s_red[2 * (tid >> 6) + 1] = b;
+        }
+        __syncthreads();
+        a = 0.0;
+        b = 0.0;
+        for (int i = 0; i < NT / 64; i++) {
+            a += s_red[2 * i];
+            b += s_red[2 * i + 1];
+        }
+    };
+
+    for (int j = tid; j < c; j += NT) {
+        double g = G[j + (long)j * c];
+        D[j] = g > 0.0 ? sqrt(g) : 0.0;
+    }
+    if (tid == 0) s_def = 0;
+    __syncthreads();
+    for (long e = tid; e < (long)c * c; e += NT) {
+        int k = (int)(e % c), i = (int)(e / c);
+        double di = D[i], dk = D[k];
+        S[e] = (di > 0.0 && dk > 0.0) ? G[k + (long)i * c] / (di * dk) : 0.0;
+    }
+    // start vector of the power iterations, unit length
+    double nrm0 = 0.0;
+    for (int i = 0; i < c; i++) nrm0 += (1.0 + 0.01 * (i % 7)) * (1.0 + 0.01 * (i % 7));
+    nrm0 = 1.0 / sqrt(nrm0);
+    for (int i = tid; i < c; i += NT) pv[i] = (1.0 + 0.01 * (i % 7)) * nrm0;
+    __syncthreads();
+    const int PIT = want_kappa ? 6 : 0;
+    const int row8 = tid >> 3, sub8 = tid & 7;
+    double lmax = 0.0;
+    for (int it = 0; it < PIT; it++) {
+        // y = S x with x of unit length: lambda ~ ||y||
+        for (int i0 = 0; i0 < c; i0 += NT / 8) {
+            int i = i0 + row8;
+            double acc = 0.0;
+            if (i < c)
+                for (int l = sub8; l < c; l += 8) acc += S[(long)i * c + l] * pv[l];
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 2);
+            acc += __shfl_xor(acc, 4);
+            if (i < c && sub8 == 0) pv[c + i] = acc;
+        }
+        __syncthreads();
+        double nn = 0.0, dummy = 0.0;
+        for (int i = tid; i < c; i += NT) nn += pv[c + i] * pv[c + i];
+        block_sum2(nn, dummy);
+        lmax = sqrt(nn);
+        double inv = nn > 0.0 ? 1.0 / lmax : 0.0;
+        for (int i = tid; i < c; i += NT) pv[i] = pv[c + i] * inv;
+        __syncthreads();
+    }
+    // Cholesky, upper triangle of S, one barrier per step.
+    const int hw = tid >> 5, hl = tid & 31;  // 32 half-waves of 32 lanes
+    for (int j = 0; j < c; j++) {
+        if (j > 0) {
+            // deferred: finish row j-1
+            const int q = j - 1;
+            const double pq = tdg[q];
+            const double rqq = pq > 0.0 ? sqrt(pq) : 0.0;
+            const double iq = pq > 0.0 ? 1.0 / rqq : 0.0;
+            for (int k = q + tid; k < c; k += NT) S[(long)q * c + k] = (k == q) ? rqq : S[(long)q * c + k] * iq;
+        }
+        const double p = S[(long)j * c + j] + shift;
+        const bool dead = !(D[j] > 0.0) || !(p > tiny);
+        if (tid == 0) {
+            tdg[j] = dead ? 0.0 : p;
+            dd[j] = dead ? 1.0 : 0.0;
+            if (dead) s_def += 1;
+        }
+        if (!dead) {
+            const double ip = 1.0 / p;
+            for (int i = j + 1 + hw; i < c; i += NT / 32) {
+                const double sji = S[(long)j * c + i] * ip;
+                for (int k = i + hl; k < c; k += 32) S[(long)i * c + k] -= sji * S[(long)j * c + k];
+            }
+        }
+        __syncthreads();
+    }
+    {
+        const int q = c - 1;
+        const double pq = tdg[q];
+        const double rqq = pq > 0.0 ? sqrt(pq) : 0.0;
+        if (tid == 0) S[(long)q * c + q] = rqq;
+    }
+    __syncthreads();
+    // Blocked inverse Ts = Rs^-1 (dead rows and columns zero).  Ts(i,k) for
+    // i < k lives at S[k*c + i]; its diagonal in tdg.
+    const int nb = (c + CHOL_BLK - 1) / CHOL_BLK;
+    for (int j = tid; j < c; j += NT) tdg[j] = (dd[j] != 0.0) ? 0.0 : 1.0 / S[(long)j * c + j];
+    __syncthreads();
+    auto tget = [&](int i, int k) -> double {  // Ts(i,k), i <= k
+        return (i == k) ? tdg[i] : S[(long)k * c + i];
+    };
+    // diagonal blocks, all in parallel: column jj of every block per step
+    for (int jj = 1; jj < CHOL_BLK; jj++) {
+        // rows i of block q with i < j = q*32 + jj; 8 lanes per (q, i)
+        const int q = row8 >> 5, il = row8 & 31;  // row8 in [0, 128)
+        const int j = q * CHOL_BLK + jj;
+        const int i = q * CHOL_BLK + il;
+        double acc = 0.0;
+        const bool act = (q < nb) && (j < c) && (il < jj);
+        if (act && dd[j] == 0.0)
+            for (int l = i + sub8; l < j; l += 8) acc += tget(i, l) * S[(long)l * c + j];
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        acc += __shfl_xor(acc, 4);
+        // writes go to row j of the lower triangle; this step reads rows < j
+        if (act && sub8 == 0) S[(long)j * c + i] = -acc * tdg[j];
+        __syncthreads();
+    }
+    // off-diagonal blocks by distance: Ts_ij = -Ts_ii * sum_{l=i+1..j} Rs_il Ts_lj
+    for (int sd = 1; sd < nb; sd++) {
+        const int npair = nb - sd;
+        for (int e = tid; e < npair * CHOL_BLK * CHOL_BLK; e += NT) {
+            const int pi = e / (CHOL_BLK * CHOL_BLK), r = e - pi * CHOL_BLK * CHOL_BLK;
+            const int a = r / CHOL_BLK, b = r - a * CHOL_BLK;
+            const int bi = pi, bj = pi + sd;
+            const int ia = bi * CHOL_BLK + a, jb = bj * CHOL_BLK + b;
+            double acc = 0.0;
+            if (ia < c && jb < c) {
+                const int l0 = (bi + 1) * CHOL_BLK;
+                for (int n = l0; n <= jb; n++) acc += S[(long)ia * c + n] * tget(n, jb);
+            }
+            Y[e] = acc;
+        }
+        __syncthreads();
+        for (int e = tid; e < npair * CHOL_BLK * CHOL_BLK; e += NT) {
+            const int pi = e / (CHOL_BLK * CHOL_BLK), r = e - pi * CHOL_BLK * CHOL_BLK;
+            const int a = r / CHOL_BLK, b = r - a * CHOL_BLK;
+            const int bi = pi, bj = pi + sd;
+            const int ia = bi * CHOL_BLK + a, jb = bj * CHOL_BLK + b;
+            if (ia < c && jb < c) {
+                double acc = 0.0;
+                const int mend = min((bi + 1) * CHOL_BLK, c);
+                for (int m = ia; m < mend; m++)
+                    acc += tget(ia, m) * Y[pi * CHOL_BLK * CHOL_BLK + (m - bi * CHOL_BLK) * CHOL_BLK + b];
+                S[(long)jb * c + ia] = -acc;
+            }
+        }
+        __syncthreads();
+    }
+    // outputs and Frobenius norms
+    double fr = 0.0, ft = 0.0;
+    for (long e = tid; e < (long)c * c; e += NT) {
+        int i = (int)(e % c), k = (int)(e / c);
+        double rv = (i <= k) ? S[(long)i * c + k] : 0.0;
+        double tv = (i <= k) ? tget(i, k) : 0.0;
+        fr += rv * rv;
+        ft += tv * tv;
+        R[e] = rv * D[k];
+        T[e] = (D[i] > 0.0) ? tv / D[i] : 0.0;
+    }
+    block_sum2(fr, ft);
+    // lambda_max(Ts Ts^T) by power iteration
+    for (int i = tid; i < c; i += NT) pv[i] = (1.0 + 0.01 * (i % 7)) * nrm0;
+    __syncthreads();
+    double lminv = 0.0;
+    for (int it = 0; it < PIT; it++) {
+        for (int k0 = 0; k0 < c; k0 += NT / 8) {  // u = Ts^T x
+            int k = k0 + row8;
+            double acc = 0.0;
+            if (k < c)
+                for (int i = sub8; i <= k; i += 8) acc += tget(i, k) * pv[i];
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 2);
+            acc += __shfl_xor(acc, 4);
+            if (k < c && sub8 == 0) pv[c + k] = acc;
+        }
+        __syncthreads();
+        for (int i0 = 0; i0 < c; i0 += NT / 8) {  // y = Ts u  (into Y scratch)
+            int i = i0 + row8;
+            double acc = 0.0;
+            if (i < c)
+                for (int k = i + sub8; k < c; k += 8) acc += tget(i, k) * pv[c + k];
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 2);
+            acc += __shfl_xor(acc, 4);
+            if (i < c && sub8 == 0) Y[i] = acc;
+        }
+        __syncthreads();
+        double nn = 0.0, dummy = 0.0;
+        for (int i = tid; i < c; i += NT) nn += Y[i] * Y[i];
+        block_sum2(nn, dummy);
+        lminv = sqrt(nn);
+        double inv = nn > 0.0 ? 1.0 / lminv : 0.0;
+        for (int i = tid; i < c; i += NT) pv[i] = Y[i] * inv;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        dinfo[0] = sqrt(fr) * sqrt(ft);
+        dinfo[1] = want_kappa ? sqrt(lmax * lminv) : 0.0;
+        dinfo[2] = (double)s_def;
+        info[0] = s_def;
+    }
+}
+
+// Rotation of a double across the 16 lanes of a DPP row (row_ror:N).
+template <int N>
+__device__ __forceinline__ double row_ror(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x120 + N, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x120 + N, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum of three values over each aligned group of G lanes (G a power of two
+// <= 64), every lane receiving the sums.  Groups of 16 and 64 use row
+// rotations (DPP) inside each 16-lane row.
+__device__ __forceinline__ void group_allreduce3(double &x, double &y, double &z, int G) {
+    if (G == 16 || G == 64) {
+        x += row_ror<8>(x); y += row_ror<8>(y); z += row_ror<8>(z);
+        x += row_ror<4>(x); y += row_ror<4>(y); z += row_ror<4>(z);
+        x += row_ror<2>(x); y += row_ror<2>(y); z += row_ror<2>(z);
+        x += row_ror<1>(x); y += row_ror<1>(y); z += row_ror<1>(z);
+        if (G == 64) {
+            for (int o = 16; o < 64; o <<= 1) {
+                x += __shfl_xor(x, o);
+                y += __shfl_xor(y, o);
+                z += __shfl_xor(z, o);
+            }
+        }
+    } else {
+        for (int o = 1; o < G; o <<= 1) {
+            x += __shfl_xor(x, o);
+            y += __shfl_xor(y, o);
+            z += __shfl_xor(z, o);
+        }
+    }
+}
+
 __device__ __forceinline__ void rr_pair(int kp, int t, int p, int &a, int &b) {
     // Round-robin (circle) schedule over kp players, round t in [0, kp-1).
     const int m = kp - 1;
@@ -243,8 +507,12 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
                                                    int *__restrict__ posout) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
-    double *W = INLDS ? lds : gwork;  // m x k, ld m
-    double *Vw = (V && !LOGROT) ? (VLDS ? lds + (long)m * k : gwork + (long)m * k) : nullptr;
+    // Columns in LDS use an odd stride (in doubles) so that the 16 pairs a
+    // wave touches start in distinct banks.
+    const int ldw = INLDS ? (m | 1) : m;
+    double *W = INLDS ? lds : gwork;  // m x k, ld ldw
+    double *Vw = (V && !LOGROT) ? (VLDS ? lds + (long)ldw * k : gwork + (long)m * k) : nullptr;
+    const int ldvw = VLDS ? (k | 1) : k;
     __shared__ int s_rot;
     __shared__ double s_sig[1024];
     __shared__ double s_sorted[1024];
@@ -254,6 +522,10 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     // rotations: nonzero columns are packed first, in their original order.
     __shared__ int s_perm[1024];
     __shared__ int s_knz;
+    // The columns are scaled by 2^-e (exact) so that the largest entry is
+    // O(1): bond matrices of deep trains reach 1e-60 and below, where the
+    // products of squared norms would underflow.
+    __shared__ int s_exp;
     for (int j = tid; j < k; j += NT) {
         double s = 0.0;
         for (int i = 0; i < m; i++) s = fmax(s, fabs(A[i + (long)j * lda]));
@@ -262,8 +534,12 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     __syncthreads();
     if (tid == 0) {
         int q = 0;
-        for (int j = 0; j < k; j++)
+        double amax = 0.0;
+        for (int j = 0; j < k; j++) {
             if (s_sig[j] > 0.0) s_perm[q++] = j;
+            amax = fmax(amax, s_sig[j]);
+        }
+        s_exp = (amax > 0.0) ? ilogb(amax) : 0;
         s_knz = q;
         for (int j = 0; j < k; j++)
             if (!(s_sig[j] > 0.0)) s_perm[q++] = j;
@@ -271,12 +547,12 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     __syncthreads();
     for (long e = tid; e < (long)m * k; e += NT) {
         long j = e / m, i = e - j * m;
-        W[e] = A[i + (long)s_perm[j] * lda];
+        W[i + j * ldw] = ldexp(A[i + (long)s_perm[j] * lda], -s_exp);
     }
     if (Vw)
         for (long e = tid; e < (long)k * k; e += NT) {
             long j = e / k, i = e - j * k;
-            Vw[e] = (i == s_perm[j]) ? 1.0 : 0.0;
+            Vw[i + j * ldvw] = (i == s_perm[j]) ? 1.0 : 0.0;
         }
     if (LOGROT)
         for (int j = tid; j < k; j += NT) posout[1024 + j] = s_perm[j];
@@ -305,9 +581,28 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
                 int a = -1, b = -1;
                 if (p < P) rr_pair(kp, t, p, a, b);
                 bool valid = (p < P) && a < knz && b < knz;
+                // Column segments of the pair stay in registers between the
+                // dot products and the rotation when they fit (m <= 8 G).
+                const bool inreg = (m <= SVD_RV * G);
+                double xa[SVD_RV], xb[SVD_RV];
                 double al = 0.0, be = 0.0, ga = 0.0;
-                if (valid) {
-                    const double *ca = W + (long)a * m, *cb = W + (long)b * m;
+                double *ca = W + (long)(valid ? a : 0) * ldw, *cb = W + (long)(valid ? b : 0) * ldw;
+                if (inreg) {
+#pragma unroll
+                    for (int q = 0; q < SVD_RV; q++) {
+                        const int i = gl + q * G;
+                        double x = 0.0, y = 0.0;
+                        if (valid && i < m) {
+                            x = ca[i];
+                            y = cb[i];
+                        }
+                        xa[q] = x;
+                        xb[q] = y;
+                        al += x * x;
+                        be += y * y;
+                        ga += x * y;
+                    }
+                } else if (valid) {
                     for (int i = gl; i < m; i += G) {
                         double x = ca[i], y = cb[i];
                         al += x * x;
@@ -315,42 +610,50 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
                         ga += x * y;
                     }
                 }
-                for (int o = 1; o < G; o <<= 1) {
-                    al += __shfl_xor(al, o);
-                    be += __shfl_xor(be, o);
-                    ga += __shfl_xor(ga, o);
-                }
-                bool rot = valid && fabs(ga) > tol * sqrt(al * be) && al > 0.0 && be > 0.0;
-                if (LOGROT && p < P && gl == 0) {
-                    double2 cslog = make_double2(1.0, 0.0);
-                    if (rot) {
-                        double zeta = (be - al) / (2.0 * ga);
-                        double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                        double cs = 1.0 / sqrt(1.0 + tt * tt);
-                        cslog = make_double2(cs, cs * tt);
-                    }
-                    rotlog[((long)sweep * (kp - 1) + t) * P + p] = cslog;
-                }
+                group_allreduce3(al, be, ga, G);
+                const double nab = sqrt(al) * sqrt(be);
+                bool rot = valid && fabs(ga) > tol * nab && al > 0.0 && be > 0.0;
+                double cs = 1.0, sn = 0.0;
                 if (rot) {
                     double zeta = (be - al) / (2.0 * ga);
                     double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
-                    double *ca = W + (long)a * m, *cb = W + (long)b * m;
-                    for (int i = gl; i < m; i += G) {
-                        double x = ca[i], y = cb[i];
-                        ca[i] = cs * x - sn * y;
-                        cb[i] = sn * x + cs * y;
+                    cs = 1.0 / sqrt(1.0 + tt * tt);
+                    sn = cs * tt;
+                }
+                // The computed cosine carries rounding up to ~m u even for
+                // exactly orthogonal columns, so rotations start above tol
+                // but only cosines above 2 tol keep the sweeps going;
+                // otherwise noise-triggered rotations never settle.
+                const bool unsettled = rot && fabs(ga) > 2.0 * tol * nab;
+                if (LOGROT && p < P && gl == 0)
+                    rotlog[((long)sweep * (kp - 1) + t) * P + p] = make_double2(cs, sn);
+                if (rot) {
+                    if (inreg) {
+#pragma unroll
+                        for (int q = 0; q < SVD_RV; q++) {
+                            const int i = gl + q * G;
+                            if (i < m) {
+                                ca[i] = cs * xa[q] - sn * xb[q];
+                                cb[i] = sn * xa[q] + cs * xb[q];
+                            }
+                        }
+                    } else {
+                        for (int i = gl; i < m; i += G) {
+                            double x = ca[i], y = cb[i];
+                            ca[i] = cs * x - sn * y;
+                            cb[i] = sn * x + cs * y;
+                        }
                     }
                     if (Vw) {
-                        double *va = Vw + (long)a * k, *vb = Vw + (long)b * k;
+                        double *va = Vw + (long)a * ldvw, *vb = Vw + (long)b * ldvw;
                         for (int i = gl; i < k; i += G) {
                             double x = va[i], y = vb[i];
                             va[i] = cs * x - sn * y;
                             vb[i] = sn * x + cs * y;
                         }
                     }
-                    rot_local = 1;
                 }
+                if (unsettled) rot_local = 1;
             }
             __syncthreads();
         }
@@ -361,7 +664,7 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     }
     // column norms
     for (int j = g; j < k; j += ngroups) {
-        const double *cj = W + (long)j * m;
+        const double *cj = W + (long)j * ldw;
         double s = 0.0;
         for (int i = gl; i < m; i += G) s += cj[i] * cj[i];
         for (int o = 1; o < G; o <<= 1) s += __shfl_xor(s, o);
@@ -380,19 +683,19 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     }
     __syncthreads();
     for (int j = tid; j < k; j += NT) {
-        sig[s_pos[j]] = s_sig[j];
-        s_sorted[s_pos[j]] = s_sig[j];
+        sig[s_pos[j]] = ldexp(s_sig[j], s_exp);
+        s_sorted[s_pos[j]] = s_sig[j];  // scaled by 2^-e
         if (LOGROT) posout[j] = s_pos[j];
     }
     for (long e = tid; e < (long)m * k; e += NT) {
         long j = e / m, i = e - j * m;
         double sj = s_sig[j];
-        U[i + (long)s_pos[j] * ldu] = sj > 0.0 ? W[e] / sj : 0.0;
+        U[i + (long)s_pos[j] * ldu] = sj > 0.0 ? W[i + j * ldw] / sj : 0.0;
     }
     if (Vw)
         for (long e = tid; e < (long)k * k; e += NT) {
             long j = e / k, i = e - j * k;
-            V[i + (long)s_pos[j] * ldv] = Vw[e];
+            V[i + (long)s_pos[j] * ldv] = Vw[i + j * ldvw];
         }
     __syncthreads();
     if (tid == 0) {
@@ -406,7 +709,8 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
         for (int q = numrank - 1; q >= 0; q--) {
             double v = s_sorted[q];
             tail += v * v;
-            double lhs = (tail > 0.0) ? 0.5 * log(tail) + log_scale : -INFINITY;
+            double lhs = (tail > 0.0) ? 0.5 * log(tail) + log_scale + s_exp * 0.6931471805599453
+                                      : -INFINITY;
             if (delta > 0.0 && lhs < ld) rho = q;
             else break;
         }
@@ -481,6 +785,7 @@ void allow_big_lds() {
     static bool done = false;
     if (done) return;
     set_lds_attr(k_chol_qr_factor<true>, LDS_DOUBLES);
+    set_lds_attr(k_chol_fast, CHOL_LDS_DOUBLES);
     set_lds_attr(k_svd_jacobi<true, true, false>, SVD_LDS_DOUBLES);
     set_lds_attr(k_svd_jacobi<true, false, true>, SVD_LDS_DOUBLES);
     done = true;
@@ -495,7 +800,11 @@ void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny, d
     ptt_hip_init();
     allow_big_lds();
     size_t need = (size_t)c * c + 5 * (size_t)c;
-    if (need <= (size_t)LDS_DOUBLES) {
+    if (c <= CHOL_FAST_C) {
+        const size_t fneed = need + 3 * CHOL_BLK * CHOL_BLK;
+        PTT_LAUNCH("chol_qr_factor", k_chol_fast, dim3(1), dim3(NT), fneed * sizeof(double), c,
+                   G, shift, tiny, R, T, info, dinfo, want_kappa);
+    } else if (need <= (size_t)LDS_DOUBLES) {
         PTT_LAUNCH("chol_qr_factor", k_chol_qr_factor<true>, dim3(1), dim3(NT),
                    need * sizeof(double), c, G, shift, tiny, R, T, info, dinfo, (double *)nullptr,
                    want_kappa);
@@ -515,13 +824,15 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *
         fprintf(stderr, "paratt_amd: Jacobi SVD limited to k <= 1024 (k = %d)\n", k);
         exit(1);
     }
-    const size_t mk = (size_t)m * k, kk = (size_t)k * k;
-    if (mk + (V ? kk : 0) <= (size_t)SVD_LDS_DOUBLES) {
+    const size_t kk = (size_t)(k | 1) * k;  // padded V footprint (LDS case)
+    const size_t mk = (size_t)m * k;
+    const size_t mkl = (size_t)(m | 1) * k;  // LDS footprint of the padded columns
+    if (mkl + (V ? kk : 0) <= (size_t)SVD_LDS_DOUBLES) {
         PTT_LAUNCH("svd_jacobi", (k_svd_jacobi<true, true, false>), dim3(1), dim3(NT),
-                   (mk + (V ? kk : 0)) * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv,
+                   (mkl + (V ? kk : 0)) * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv,
                    log_scale, delta, cap, iout, (double *)nullptr, (double2 *)nullptr,
                    (int *)nullptr);
-    } else if (mk <= (size_t)SVD_LDS_DOUBLES) {
+    } else if (mkl <= (size_t)SVD_LDS_DOUBLES) {
         // columns in LDS; V (if wanted) from the rotation log
         const int kp = k + (k & 1), P = kp / 2;
         const size_t logn = (size_t)40 * (kp - 1) * P;
@@ -529,7 +840,7 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *
         double2 *rotlog = (double2 *)ws;
         int *pos = (int *)(ws + logn * sizeof(double2));
         PTT_LAUNCH("svd_jacobi", (k_svd_jacobi<true, false, true>), dim3(1), dim3(NT),
-                   mk * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv, log_scale, delta, cap,
+                   mkl * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv, log_scale, delta, cap,
                    iout, (double *)nullptr, rotlog, pos);
         if (V)
             PTT_LAUNCH("svd_replay_v", k_svd_replay_v, dim3(k), dim3(64), 0, k, rotlog, iout + 3,

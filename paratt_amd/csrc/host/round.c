@@ -31,6 +31,7 @@
 #include "ptt_host.h"
 
 #define KAPPA_ONE_PASS 30.0
+#define CHOLQR_MAX_PASSES 6
 #define KAPPA_TWO_PASS 1.0e6
 
 static paratt_amd_round_stats g_stats;
@@ -152,36 +153,48 @@ static int cholqr_core(attac_tt *x, int core, char trans, long len, int c, doubl
         *Rout = w->R1;
         return 1;
     }
-    double shift = 0.0;
-    int passes = 2;
+    /* Multi-pass Cholesky-QR that never drops content: a pass whose Gram is
+     * deficient (a pivot below the guard) or too ill-conditioned is factored
+     * with the Fukaya et al. shift instead.  A shifted pass still satisfies
+     * A = Q R exactly and only leaves Q less orthogonal, so passes continue
+     * until the last one started from a well-conditioned Q: two unshifted
+     * passes after a shifted one, one after a moderately conditioned one,
+     * at most CHOLQR_MAX_PASSES.  (Dropping a deficient pivot instead would
+     * discard up to sqrt(guard) of that row's content.) */
+    const double shift = cholqr_shift(len, c);
+    int need;
     if (!(o.deficient == 0 && o.kappa <= KAPPA_TWO_PASS)) {
-        shift = cholqr_shift(len, c);
         o = run_chol(c, w->G, shift, w->R1, w->T1, w);
-        passes = 3;
+        need = 2;
+    } else {
+        need = 1;
     }
+    int passes = 1;
     /* pass 1 */
     if (trans == 'T') apply_right(x, core, len, c, w->T1, 1.0);
     else apply_left_t(x, core, c, len, w->T1, 1.0);
-    const double *Rprev = w->R1;
     double *Racc = w->Racc;
     ptt_hip_d2d(Racc, w->R1, sizeof(double) * (size_t)c * c);
-    for (int p = 2; p <= passes; p++) {
+    while (need > 0) {
+        passes++;
         A = x->coreList[core];
         ptt_hip_gram(trans, c, len, A, trans == 'T' ? len : c, w->G);
-        double *Rp = (p == 2) ? w->R2 : w->R3;
-        double *Tp = (p == 2) ? w->T2 : w->T3;
+        double *Rp = (passes % 2 == 0) ? w->R2 : w->R3;
+        double *Tp = (passes % 2 == 0) ? w->T2 : w->T3;
         chol_out q = run_chol(c, w->G, 0.0, Rp, Tp, w);
-        if (q.deficient != 0 && p == passes) {
-            /* the last pass must not lose columns: shift it */
-            run_chol(c, w->G, cholqr_shift(len, c), Rp, Tp, w);
+        if (q.deficient != 0 || q.kappa > KAPPA_TWO_PASS) {
+            run_chol(c, w->G, shift, Rp, Tp, w);
+            need = 2;
+        } else {
+            need = (q.kappa <= KAPPA_ONE_PASS) ? 0 : 1;
         }
-        double a = (p == passes) ? qalpha : 1.0;
+        if (passes >= CHOLQR_MAX_PASSES) need = 0;
+        const double a = (need == 0) ? qalpha : 1.0;
         if (trans == 'T') apply_right(x, core, len, c, Tp, a);
         else apply_left_t(x, core, c, len, Tp, a);
         /* Racc <- Rp * Racc */
         ptt_hip_trmm_small(c, Rp, Racc, w->Rtmp);
         ptt_hip_d2d(Racc, w->Rtmp, sizeof(double) * (size_t)c * c);
-        (void)Rprev;
     }
     *Rout = Racc;
     return passes;

@@ -1,0 +1,181 @@
+"""GPU small dense kernels (Jacobi-scaled Cholesky-QR factor, one-sided
+Jacobi SVD) against numpy/LAPACK on the same matrices.  Tolerances are
+stated per check: backward-stable factorizations of well-conditioned inputs
+agree to 1e-12 relative."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import paratt_amd as P
+from gpu_util import DevBuf
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    L = P.lib()
+    L.ptt_hip_chol_qr_factor.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_int]
+    L.ptt_hip_svd_jacobi.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_long, C.c_void_p,
+                                     C.c_void_p, C.c_long, C.c_void_p, C.c_long, C.c_double,
+                                     C.c_double, C.c_int, C.c_void_p]
+    return L
+
+
+def _ints(buf, n):
+    return np.frombuffer(buf.get().tobytes(), dtype=np.int32)[:n]
+
+
+def _chol(G, shift=0.0, tiny=1e-14, kappa=1):
+    L = _lib()
+    c = G.shape[0]
+    dG = DevBuf(G.ravel(order="F"))
+    dR, dT = DevBuf(np.zeros(c * c)), DevBuf(np.zeros(c * c))
+    di, dd = DevBuf(np.zeros(2)), DevBuf(np.zeros(4))
+    L.ptt_hip_chol_qr_factor(c, dG.ptr, shift, tiny, dR.ptr, dT.ptr, di.ptr, dd.ptr, kappa)
+    L.ptt_hip_sync()
+    R = dR.get().reshape(c, c, order="F")
+    T = dT.get().reshape(c, c, order="F")
+    info = int(_ints(di, 1)[0])
+    dinfo = dd.get()
+    for b in (dG, dR, dT, di, dd):
+        b.free()
+    return R, T, info, dinfo
+
+
+@pytest.mark.parametrize("c", [1, 7, 32, 33, 64, 100, 128, 129, 160])
+def test_chol_matches_lapack(c):
+    rng = np.random.default_rng(c)
+    A = rng.standard_normal((3 * c + 5, c)) * np.logspace(0, 3, c)  # graded columns
+    G = A.T @ A
+    R, T, info, dinfo = _chol(G)
+    assert info == 0 and dinfo[2] == 0
+    Rref = np.linalg.cholesky(G).T
+    assert np.allclose(np.tril(R, -1), 0) and np.allclose(np.tril(T, -1), 0)
+    # column-scaled relative error (the factor inherits the grading)
+    colsc = np.sqrt(np.diag(G))
+    assert np.max(np.abs(R - Rref) / colsc[None, :]) < 1e-12
+    assert np.max(np.abs(T @ R - np.eye(c))) < 1e-12
+    Q = A @ T
+    assert np.max(np.abs(Q.T @ Q - np.eye(c))) < 1e-11
+    # condition estimates: power iteration converges from below
+    D = np.sqrt(np.diag(G))
+    Rs = Rref / D[None, :]
+    ks = np.linalg.cond(Rs)
+    assert 0.3 * ks <= dinfo[1] <= 1.0001 * ks
+    assert dinfo[0] >= 0.999 * ks  # Frobenius product bounds the 2-norm one
+
+
+@pytest.mark.parametrize("c", [6, 64, 128, 150])
+def test_chol_deficient_columns_marked(c):
+    rng = np.random.default_rng(10 + c)
+    A = rng.standard_normal((2 * c, c))
+    A[:, 3] = A[:, 1]            # exact duplicate
+    A[:, c - 1] = 2.0 * A[:, 0]  # exact multiple
+    G = A.T @ A
+    R, T, info, dinfo = _chol(G)
+    assert info == 2 and dinfo[2] == 2
+    assert np.all(R[3, :] == 0) and np.all(R[c - 1, :] == 0)
+    assert np.all(T[:, 3] == 0) and np.all(T[:, c - 1] == 0)
+    Q = A @ T
+    live = [j for j in range(c) if j not in (3, c - 1)]
+    assert np.max(np.abs(Q[:, live].T @ Q[:, live] - np.eye(len(live)))) < 1e-11
+    assert np.max(np.abs(Q @ R - A)) < 1e-12 * np.max(np.abs(A))
+
+
+@pytest.mark.parametrize("c", [16, 128])
+def test_chol_shifted(c):
+    rng = np.random.default_rng(20 + c)
+    A = rng.standard_normal((2 * c, c))
+    A[:, 5] = A[:, 4] + 1e-9 * rng.standard_normal(2 * c)
+    G = A.T @ A
+    s = 1e-6
+    R, T, info, _ = _chol(G, shift=s, tiny=0.0, kappa=0)
+    assert info == 0
+    D = np.sqrt(np.diag(G))
+    Rref = np.linalg.cholesky(G / np.outer(D, D) + s * np.eye(c)).T * D[None, :]
+    assert np.max(np.abs(R - Rref) / D[None, :]) < 1e-12
+    assert np.max(np.abs(T @ R - np.eye(c))) < 1e-9
+
+
+def _svd(A, wantV, delta=0.0, log_scale=0.0, cap=None):
+    L = _lib()
+    m, k = A.shape
+    cap = k if cap is None else cap
+    dA = DevBuf(A.ravel(order="F"))
+    ds, dU = DevBuf(np.zeros(k)), DevBuf(np.zeros(m * k))
+    dV = DevBuf(np.zeros(k * k))
+    io = DevBuf(np.zeros(4))
+    L.ptt_hip_svd_jacobi(m, k, dA.ptr, m, ds.ptr, dU.ptr, m, dV.ptr if wantV else None, k,
+                         log_scale, delta, cap, io.ptr)
+    L.ptt_hip_sync()
+    s, U = ds.get(), dU.get().reshape(m, k, order="F")
+    V = dV.get().reshape(k, k, order="F") if wantV else None
+    iv = _ints(io, 5)
+    Aafter = dA.get().reshape(m, k, order="F")
+    for b in (dA, ds, dU, dV, io):
+        b.free()
+    return s, U, V, iv, Aafter
+
+
+def _tail_rank(s, delta, numrank, cap):
+    """pattac.c:468-481: smallest j with sqrt(sum_{l>=j, l<numrank} s_l^2) < delta."""
+    rho, tail = numrank, 0.0
+    for q in range(numrank - 1, -1, -1):
+        tail += s[q] ** 2
+        if np.sqrt(tail) < delta:
+            rho = q
+        else:
+            break
+    return max(1, min(rho, len(s), cap))
+
+
+@pytest.mark.parametrize("m,k,wantV", [(5, 3, True), (32, 32, True), (64, 40, True),
+                                       (128, 128, False), (128, 128, True), (129, 64, True),
+                                       (200, 100, True), (300, 17, False)])
+def test_svd_matches_lapack(m, k, wantV):
+    rng = np.random.default_rng(m * 1000 + k)
+    Q1, _ = np.linalg.qr(rng.standard_normal((m, k)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    sv = np.logspace(0, -10, k)
+    A = (Q1 * sv) @ Q2.T
+    s, U, V, iv, Aafter = _svd(A, wantV)
+    assert np.array_equal(Aafter, A)
+    sref = np.linalg.svd(A, compute_uv=False)
+    assert np.all(np.diff(s) <= 0)
+    assert np.max(np.abs(s - sref)) < 1e-13 * sref[0] * max(1, np.sqrt(k) / 4)
+    assert np.max(np.abs(U.T @ U - np.eye(k))) < 1e-10  # tiny sigma: U from A v / sigma
+    if wantV:
+        assert np.max(np.abs(V.T @ V - np.eye(k))) < 1e-13 * k
+        assert np.max(np.abs((U * s) @ V.T - A)) < 1e-13 * k
+    assert iv[0] == np.sum(s > np.sqrt(k) * 2.0 ** -53 * s[0])
+
+
+@pytest.mark.parametrize("k", [20, 128])
+def test_svd_truncation_rank_rule(k):
+    rng = np.random.default_rng(k)
+    Q1, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    sv = np.logspace(0, -12, k)
+    A = (Q1 * sv) @ Q2.T
+    for delta in (1e-3, 1e-7, 1e-11):
+        for cap in (k, k // 3):
+            s, _, _, iv, _ = _svd(A, False, delta=delta, cap=cap)
+            assert iv[1] == _tail_rank(s, delta, int(iv[0]), cap), (delta, cap)
+    # log_scale shifts the spectrum before the rule
+    s, _, _, iv, _ = _svd(A, False, delta=1e-7 * 8.0, log_scale=np.log(8.0))
+    assert iv[1] == _tail_rank(s, 1e-7, int(iv[0]), k)
+
+
+def test_svd_zero_columns_and_rank_deficiency():
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((40, 30))
+    A[:, 4] = 0.0
+    A[:, 17] = 0.0
+    s, U, V, iv, _ = _svd(A, True)
+    sref = np.linalg.svd(A, compute_uv=False)
+    assert np.max(np.abs(s - sref)) < 1e-13 * sref[0] * 4
+    assert iv[0] == 28 and s[28] == 0.0 and s[29] == 0.0
+    assert np.max(np.abs((U * s) @ V.T - A)) < 1e-12 * 30

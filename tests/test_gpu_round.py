@@ -65,3 +65,60 @@ def test_round_relative_tolerance_equals_norm_then_truncate(seed):
     out = to_host(dz)
     assert out.r == ref.r
     assert O.rel_diff(out, ref) < 1e-12 + 1e-7
+
+
+# Boundary shapes: r_1 > n_0 (left unfolding of core 0 wider than tall: the
+# orthogonalization keeps Q = I), r_{d-1} > n_{d-1} (the last right unfolding
+# has fewer columns than rows: SVD of the core itself), and both together.
+@pytest.mark.parametrize("n,rx", [
+    ([4, 16, 16, 16], [1, 4, 5, 5, 1]),
+    ([16, 16, 16, 4], [1, 5, 5, 4, 1]),
+    ([4, 16, 16, 4], [1, 4, 5, 4, 1]),
+    ([32, 32, 32, 32, 32, 32], [1, 20, 24, 24, 24, 20, 1]),
+])
+@pytest.mark.parametrize("eps", [1e-10, 1e-6])
+def test_round_boundary_shapes(n, rx, eps):
+    x = O.gauss_tt(n, rx, 31)
+    y = O.gauss_tt(n, rx, 32)
+    z = O.axpby(1.0, x, 1e-7, y)  # formal ranks 2*rx, spectrum with a 1e-7 step
+    ref = O.truncate(z, eps * O.norm(z), z.r)
+    dz = P.DevTT.from_host(z)
+    P.truncate_rel(dz, eps, dz.r)
+    out = to_host(dz)
+    assert out.r == ref.r
+    assert O.rel_diff(out, ref) < 1e-12 + eps
+
+
+def test_round_rank_deficient_nongraded_bond():
+    """A Hadamard product from the reaction-Burgers field: formal rank 35 > n_0
+    = 32 at both ends and numerically rank-deficient bonds whose rows are not
+    graded; every orthogonalization pass must keep the content of nearly
+    dependent rows."""
+    d, n = 6, 32
+    h = 2 * np.pi / n
+    nu, dt, eps, cap = 0.01, 1e-3, 1e-8, 64
+    mr = [1] + [cap] * (d - 1) + [1]
+
+    def tr(z):
+        return O.truncate(z, eps * O.norm(z), mr)
+
+    ones = O.HostTT([n] * d, [1] * (d + 1), [np.ones(n) for _ in range(d)])
+    d1 = np.array([-0.5 / h, 0.0, 0.5 / h])
+    d2 = np.array([1.0, -2.0, 1.0]) / h ** 2
+
+    def rhs(u):
+        s1, lap = O.mode_sum(u, d1), O.mode_sum(u, d2)
+        v = tr(O.axpby(1.0, O.axpby(1.0, ones, -1.0, u), -1.0, s1))
+        return tr(O.axpby(1.0, tr(O.hadamard(u, v)), nu, lap))
+
+    u0 = O.rxn_burgers_rk4(O.gaussian_ic(d, n), nu, h, dt, eps, cap, 1)
+    u3 = tr(O.axpby(1.0, u0, 0.5 * dt, rhs(tr(O.axpby(1.0, u0, 0.5 * dt, rhs(u0))))))
+    v = tr(O.axpby(1.0, O.axpby(1.0, ones, -1.0, u3), -1.0, O.mode_sum(u3, d1)))
+    z = O.hadamard(u3, v)
+    assert z.r[1] > n
+    ref = O.truncate(z, eps * O.norm(z), mr)
+    dz = P.DevTT.from_host(z)
+    P.truncate_rel(dz, eps, mr)
+    out = to_host(dz)
+    assert out.r == ref.r
+    assert O.rel_diff(out, ref) < 1e-12 + eps
