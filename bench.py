@@ -147,6 +147,18 @@ def main():
                 "traffic": None,
                 "avg_launch_us": round(1000.0 * dom["ms"] / max(dom["launches"], 1), 2)}
 
+    # the GEMM family together (Gram, apply and contraction products): the
+    # MFMA-bound part of the rounding, reported beside the dominant kernel
+    gem = [k for k in kern if k["name"].startswith("dgemm") or k["name"] == "splitk_reduce"]
+    gms = sum(k["ms"] for k in gem)
+    gfl = sum(k["flops"] for k in gem)
+    roof_gemm = None
+    if gms > 0:
+        ag = gfl / (gms * 1e-3) / 1e12
+        roof_gemm = {"bound": "mfma", "kernels": "dgemm_*+splitk_reduce", "achieved": round(ag, 2),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(ag / FP64_PEAK_TFLOPS, 4), "ms_per_step": round(gms / a.steps, 3)}
+
     # whole-rounding algorithmic model (flops of the reference's algorithm and
     # its minimal HBM traffic: read input 3x, write output once)
     s_in = sum(z0.r[k] * n * z0.r[k + 1] for k in range(d))
@@ -186,6 +198,7 @@ def main():
                                % (d, n, r, r, 2 * r, a.cap, a.eps),
                    "precision": "fp64-plain", "ranks_out": ranks[:3] + ["..."] + ranks[-2:]},
         "roofline": roof,
+        "roofline_gemm": roof_gemm,
         "cpu_baseline": cpu,
         "model": {"flops": flops_model, "bytes": bytes_model,
                   "fp64_tflops": round(eff_tflops, 2),

@@ -24,7 +24,8 @@ constexpr int LDS_DOUBLES = 17408;  // 136 KB dynamic + static arrays <= 160 KB 
 // The Jacobi kernel holds 24 KB of static LDS (sorting and packing tables),
 // so its dynamic budget is smaller: 17280 doubles + 24.1 KB < 160 KB.
 constexpr int SVD_LDS_DOUBLES = 17280;
-constexpr int SVD_RV = 8;  // rows per lane held in registers by a Jacobi pair
+constexpr int SVD_NT = 1024;  // Jacobi SVD workgroup: 16 waves, four per SIMD
+constexpr int SVD_RV = 8;     // rows per lane held in registers by a Jacobi pair
 
 // Work matrix S is stored ROW-major in LDS (S[i*c + k] = entry (i,k)) so the
 // pivot row read by every thread in the trailing update is contiguous and
@@ -481,6 +482,69 @@ __device__ __forceinline__ void group_allreduce3(double &x, double &y, double &z
     }
 }
 
+// DPP move of a double inside a quad / row (dpp_ctrl CTRL).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum of three values over each aligned group of G lanes (G a power of two
+// <= 64), every lane receiving the sums: quad permutations (xor 1, xor 2),
+// then the half-row and row mirrors, then cross-row shuffles.
+__device__ __forceinline__ void group_allreduce3_q(double &x, double &y, double &z, int G) {
+    if (G >= 2) { x += dpp_d<0xB1>(x); y += dpp_d<0xB1>(y); z += dpp_d<0xB1>(z); }
+    if (G >= 4) { x += dpp_d<0x4E>(x); y += dpp_d<0x4E>(y); z += dpp_d<0x4E>(z); }
+    if (G >= 8) { x += dpp_d<0x141>(x); y += dpp_d<0x141>(y); z += dpp_d<0x141>(z); }
+    if (G >= 16) { x += dpp_d<0x140>(x); y += dpp_d<0x140>(y); z += dpp_d<0x140>(z); }
+    for (int o = 16; o < G; o <<= 1) {
+        x += __shfl_xor(x, o);
+        y += __shfl_xor(y, o);
+        z += __shfl_xor(z, o);
+    }
+}
+
+// Round-robin pairing as rr_pair, without integer division.
+__device__ __forceinline__ void rr_pair_fast(int kp, int t, int p, int &a, int &b) {
+    const int m = kp - 1;
+    if (p == 0) {
+        a = m;
+        b = t;
+    } else {
+        a = t + p;
+        if (a >= m) a -= m;
+        b = t - p;
+        if (b < 0) b += m;
+    }
+}
+
+// Jacobi rotation (c, s) zeroing the off-diagonal of [[al, ga], [ga, be]].
+// The tangent only steers convergence and is formed from the hardware
+// reciprocal / reciprocal-square-root approximations; c = 1/sqrt(1+t^2) is
+// refined by two Newton steps so that c^2 + s^2 = 1 to working accuracy,
+// which is what keeps the accumulated transformations orthogonal.
+__device__ __forceinline__ void jacobi_cs(double al, double be, double ga, double &c, double &s) {
+    const double zeta = (be - al) * 0.5 * __builtin_amdgcn_rcp(ga);
+    const double az = fabs(zeta);
+    double t;
+    if (az > 1.0e150) {
+        t = 0.5 * __builtin_amdgcn_rcp(zeta);
+    } else {
+        const double q = 1.0 + zeta * zeta;
+        const double root = q * __builtin_amdgcn_rsq(q);  // ~sqrt(q)
+        t = __builtin_amdgcn_rcp(az + root);
+        if (zeta < 0.0) t = -t;
+    }
+    const double w = 1.0 + t * t;
+    double y = __builtin_amdgcn_rsq(w);
+    y = y * (1.5 - 0.5 * w * y * y);
+    y = y * (1.5 - 0.5 * w * y * y);
+    c = y;
+    s = y * t;
+}
+
 __device__ __forceinline__ void rr_pair(int kp, int t, int p, int &a, int &b) {
     // Round-robin (circle) schedule over kp players, round t in [0, kp-1).
     const int m = kp - 1;
@@ -497,7 +561,7 @@ __device__ __forceinline__ void rr_pair(int kp, int t, int p, int &a, int &b) {
 // kept here; every pair's rotation (c, s) is logged and replayed afterwards
 // on the rows of V by k_svd_replay_v (rows of V evolve independently).
 template <bool INLDS, bool VLDS, bool LOGROT>
-__global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *__restrict__ A,
+__global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const double *__restrict__ A,
                                                    long lda, double *__restrict__ sig,
                                                    double *__restrict__ U, long ldu,
                                                    double *__restrict__ V, long ldv,
@@ -526,7 +590,7 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     // O(1): bond matrices of deep trains reach 1e-60 and below, where the
     // products of squared norms would underflow.
     __shared__ int s_exp;
-    for (int j = tid; j < k; j += NT) {
+    for (int j = tid; j < k; j += SVD_NT) {
         double s = 0.0;
         for (int i = 0; i < m; i++) s = fmax(s, fabs(A[i + (long)j * lda]));
         s_sig[j] = s;
@@ -545,25 +609,25 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
             if (!(s_sig[j] > 0.0)) s_perm[q++] = j;
     }
     __syncthreads();
-    for (long e = tid; e < (long)m * k; e += NT) {
+    for (long e = tid; e < (long)m * k; e += SVD_NT) {
         long j = e / m, i = e - j * m;
         W[i + j * ldw] = ldexp(A[i + (long)s_perm[j] * lda], -s_exp);
     }
     if (Vw)
-        for (long e = tid; e < (long)k * k; e += NT) {
+        for (long e = tid; e < (long)k * k; e += SVD_NT) {
             long j = e / k, i = e - j * k;
             Vw[i + j * ldvw] = (i == s_perm[j]) ? 1.0 : 0.0;
         }
     if (LOGROT)
-        for (int j = tid; j < k; j += NT) posout[1024 + j] = s_perm[j];
+        for (int j = tid; j < k; j += SVD_NT) posout[1024 + j] = s_perm[j];
     __syncthreads();
     const int knz = s_knz;
 
     const int kp = knz + (knz & 1);
     const int P = kp / 2;
     int G = 64;
-    while (G > 1 && G * P > NT) G >>= 1;
-    const int ngroups = NT / G;
+    while (G > 1 && G * P > SVD_NT) G >>= 1;
+    const int ngroups = SVD_NT / G;
     const int g = tid / G, gl = tid - g * G;
     // Rotate only when the pair's cosine exceeds the rounding noise of an
     // m-term dot product; below that the columns are orthogonal to working
@@ -579,7 +643,7 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
             for (int p0 = 0; p0 < P; p0 += ngroups) {
                 int p = p0 + g;
                 int a = -1, b = -1;
-                if (p < P) rr_pair(kp, t, p, a, b);
+                if (p < P) rr_pair_fast(kp, t, p, a, b);
                 bool valid = (p < P) && a < knz && b < knz;
                 // Column segments of the pair stay in registers between the
                 // dot products and the rotation when they fit (m <= 8 G).
@@ -610,16 +674,11 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
                         ga += x * y;
                     }
                 }
-                group_allreduce3(al, be, ga, G);
+                group_allreduce3_q(al, be, ga, G);
                 const double nab = sqrt(al) * sqrt(be);
                 bool rot = valid && fabs(ga) > tol * nab && al > 0.0 && be > 0.0;
                 double cs = 1.0, sn = 0.0;
-                if (rot) {
-                    double zeta = (be - al) / (2.0 * ga);
-                    double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    cs = 1.0 / sqrt(1.0 + tt * tt);
-                    sn = cs * tt;
-                }
+                if (rot) jacobi_cs(al, be, ga, cs, sn);
                 // The computed cosine carries rounding up to ~m u even for
                 // exactly orthogonal columns, so rotations start above tol
                 // but only cosines above 2 tol keep the sweeps going;
@@ -672,7 +731,7 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
     }
     __syncthreads();
     // descending order by rank counting (ties by index)
-    for (int j = tid; j < k; j += NT) {
+    for (int j = tid; j < k; j += SVD_NT) {
         double sj = s_sig[j];
         int pos = 0;
         for (int i = 0; i < k; i++) {
@@ -682,18 +741,18 @@ __global__ __launch_bounds__(NT) void k_svd_jacobi(int m, int k, const double *_
         s_pos[j] = pos;
     }
     __syncthreads();
-    for (int j = tid; j < k; j += NT) {
+    for (int j = tid; j < k; j += SVD_NT) {
         sig[s_pos[j]] = ldexp(s_sig[j], s_exp);
         s_sorted[s_pos[j]] = s_sig[j];  // scaled by 2^-e
         if (LOGROT) posout[j] = s_pos[j];
     }
-    for (long e = tid; e < (long)m * k; e += NT) {
+    for (long e = tid; e < (long)m * k; e += SVD_NT) {
         long j = e / m, i = e - j * m;
         double sj = s_sig[j];
         U[i + (long)s_pos[j] * ldu] = sj > 0.0 ? W[i + j * ldw] / sj : 0.0;
     }
     if (Vw)
-        for (long e = tid; e < (long)k * k; e += NT) {
+        for (long e = tid; e < (long)k * k; e += SVD_NT) {
             long j = e / k, i = e - j * k;
             V[i + (long)s_pos[j] * ldv] = Vw[i + j * ldvw];
         }
@@ -802,7 +861,8 @@ void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny, d
     size_t need = (size_t)c * c + 5 * (size_t)c;
     if (c <= CHOL_FAST_C) {
         const size_t fneed = need + 3 * CHOL_BLK * CHOL_BLK;
-        PTT_LAUNCH("chol_qr_factor", k_chol_fast, dim3(1), dim3(NT), fneed * sizeof(double), c,
+        PTT_LAUNCH_FB("chol_qr_factor", 2.0 * c * (double)c * c / 3.0, 8.0 * 3.0 * c * (double)c,
+                   k_chol_fast, dim3(1), dim3(NT), fneed * sizeof(double), c,
                    G, shift, tiny, R, T, info, dinfo, want_kappa);
     } else if (need <= (size_t)LDS_DOUBLES) {
         PTT_LAUNCH("chol_qr_factor", k_chol_qr_factor<true>, dim3(1), dim3(NT),
@@ -824,11 +884,16 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *
         fprintf(stderr, "paratt_amd: Jacobi SVD limited to k <= 1024 (k = %d)\n", k);
         exit(1);
     }
+    // algorithmic work of a dense SVD with left vectors (Golub-Reinsch
+    // count 4 m k^2 + 8 k^3; right vectors are counted by their replay)
+    const double svd_flops = 4.0 * m * (double)k * k + 8.0 * (double)k * k * k;
+    const double svd_bytes = 8.0 * (2.0 * m * (double)k + k);
     const size_t kk = (size_t)(k | 1) * k;  // padded V footprint (LDS case)
     const size_t mk = (size_t)m * k;
     const size_t mkl = (size_t)(m | 1) * k;  // LDS footprint of the padded columns
     if (mkl + (V ? kk : 0) <= (size_t)SVD_LDS_DOUBLES) {
-        PTT_LAUNCH("svd_jacobi", (k_svd_jacobi<true, true, false>), dim3(1), dim3(NT),
+        PTT_LAUNCH_FB("svd_jacobi", svd_flops + (V ? 9.0 * k * (double)k * k : 0.0), svd_bytes,
+                   (k_svd_jacobi<true, true, false>), dim3(1), dim3(SVD_NT),
                    (mkl + (V ? kk : 0)) * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv,
                    log_scale, delta, cap, iout, (double *)nullptr, (double2 *)nullptr,
                    (int *)nullptr);
@@ -839,15 +904,18 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *
         char *ws = (char *)ptt_hip_scratch(PTT_WS_SKETCH, logn * sizeof(double2) + 2048 * sizeof(int));
         double2 *rotlog = (double2 *)ws;
         int *pos = (int *)(ws + logn * sizeof(double2));
-        PTT_LAUNCH("svd_jacobi", (k_svd_jacobi<true, false, true>), dim3(1), dim3(NT),
+        PTT_LAUNCH_FB("svd_jacobi", svd_flops, svd_bytes, (k_svd_jacobi<true, false, true>),
+                   dim3(1), dim3(SVD_NT),
                    mkl * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv, log_scale, delta, cap,
                    iout, (double *)nullptr, rotlog, pos);
         if (V)
-            PTT_LAUNCH("svd_replay_v", k_svd_replay_v, dim3(k), dim3(64), 0, k, rotlog, iout + 3,
+            PTT_LAUNCH_FB("svd_replay_v", 9.0 * k * (double)k * k, 8.0 * k * (double)k,
+                          k_svd_replay_v, dim3(k), dim3(64), 0, k, rotlog, iout + 3,
                        pos, V, ldv);
     } else {
         double *gwork = (double *)ptt_hip_scratch(PTT_WS_SKETCH, (mk + (V ? kk : 0)) * sizeof(double));
-        PTT_LAUNCH("svd_jacobi", (k_svd_jacobi<false, false, false>), dim3(1), dim3(NT), 0, m, k,
+        PTT_LAUNCH_FB("svd_jacobi", svd_flops + (V ? 9.0 * k * (double)k * k : 0.0), svd_bytes,
+                   (k_svd_jacobi<false, false, false>), dim3(1), dim3(SVD_NT), 0, m, k,
                    A, lda, sig, U, ldu, V, ldv, log_scale, delta, cap, iout, gwork,
                    (double2 *)nullptr, (int *)nullptr);
     }

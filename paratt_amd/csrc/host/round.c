@@ -372,6 +372,7 @@ static void rtl_sweep(attac_tt *x, const int *maxr, double abs_tol, const double
     int d = x->numDim;
     int *r = x->ttRank, *n = x->dim;
     const double delta = (d > 1) ? abs_tol / sqrt(d - 1.0) : abs_tol;
+    int prev_robust = 0;
     for (int i = d - 1; i > 0; i--) {
         int rows = r[i];
         long w = (long)n[i] * r[i + 1];
@@ -395,12 +396,18 @@ static void rtl_sweep(attac_tt *x, const int *maxr, double abs_tol, const double
             ptt_hip_svd_factors(rows, rho, ws.iinfo + 2, ws.U, rows, ws.sig, 1.0, NULL, ws.US);
         } else {
             ws = get_ws(rows);
-            rho = rtl_step_fast(x, i, maxr, delta, nscale, &ws);
+            /* Bond spectra change slowly along the train: once the Gram
+             * route was rejected at the right neighbour, go straight to the
+             * orthogonal route (saves a Gram, a factorization and an SVD). */
+            rho = prev_robust ? -1 : rtl_step_fast(x, i, maxr, delta, nscale, &ws);
             if (i < 64) g_stats.rtl_cholqr_passes[i] = 1;
+            prev_robust = (rho < 0);
             if (rho < 0) {
                 int passes = 0;
                 rho = rtl_step_robust(x, i, maxr, delta, nscale, &ws, &passes);
                 if (i < 64) g_stats.rtl_cholqr_passes[i] = 10 + passes;
+                /* a well-conditioned bond lets the next one try the Gram route */
+                if (i < 64 && g_stats.kappa_rtl[i] <= 1.0e2) prev_robust = 0;
             }
         }
         if (i < 64) {
