@@ -43,6 +43,25 @@ typedef struct {
 } paratt_amd_round_stats;
 void paratt_amd_last_round_stats(paratt_amd_round_stats *out);
 
+/* Spectrum record of the last rounding, 64 bonds x 20 doubles (bond i at
+ * out[20 i], i = 1..d-1): [0..15] sigma_{rho-13..rho+2} of the bond (0 where
+ * outside the computed spectrum), [16] sigma_0, [17] the kept rank rho,
+ * [18] the incoming rank, [19] 1 when a sketched step set the bond.
+ * Values are in the rounding's scaled space (cores times 1/sqrt(n_k)). */
+void paratt_amd_last_round_spectra(double *out);
+
+/* paratt_amd_truncate_rel with per-bond predicted ranks (pred_rank[i] for
+ * bond i = 1..d-1; <= 0 means no prediction).  A bond with a prediction p
+ * first tries a randomized range finder with p + 8 columns and one power
+ * iteration; the step is kept only when its rank provably equals the plain
+ * method's (see the certification in round.c), otherwise the plain step
+ * runs.  No counterpart in the reference (its truncation always takes the
+ * full SVD). */
+void paratt_amd_truncate_rel_sketch(paratt *x, double eps_rel, int *maxRank,
+                                    const int *pred_rank);
+/* {attempted, accepted} sketched steps of the last rounding. */
+void paratt_amd_last_sketch_counts(int *out);
+
 /* S = sum_i D_i u for a 3-point periodic stencil D applied along every mode
  * (paratt.c:665-770 semantics), as an exact rank-2r TT:
  *   core 0 [D u_0 | u_0], core k [[u_k, 0], [D u_k, u_k]], core d-1
@@ -71,4 +90,19 @@ void paratt_amd_sync(void);
 #ifdef __cplusplus
 }
 #endif
+/* ---- learned rank prediction in the step-truncation composites ----------
+ * Each truncation site of paratt_amd_rxn_burgers_field / paratt_amd_rk4_st
+ * keeps the spectrum record of its last rounding.  Learned mode predicts
+ * every bond's rank from that record with the rank-predictor network
+ * (ptt_hip_mlp_*; parameters from paratt_amd_learned_set_params) and calls
+ * paratt_amd_truncate_rel_sketch; logging mode stores (18 features, new
+ * rank / 64) pairs of plain roundings.  No counterpart in the reference. */
+void paratt_amd_learned_set_params(const float *P, int nparam);
+void paratt_amd_learned_mode(int on);
+void paratt_amd_learned_reset_history(void);
+void paratt_amd_learned_log(int on, int capacity);
+int paratt_amd_learned_log_fetch(float *X, float *Y, int max);
+/* cumulative {roundings with predictions, sketched steps tried, accepted} */
+void paratt_amd_learned_counts(long *out);
+
 #endif

@@ -571,6 +571,7 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
                                                    int *__restrict__ posout) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
+    const int nth = blockDim.x;  // chosen by the launcher for the problem size
     // Columns in LDS use an odd stride (in doubles) so that the 16 pairs a
     // wave touches start in distinct banks.
     const int ldw = INLDS ? (m | 1) : m;
@@ -590,7 +591,7 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
     // O(1): bond matrices of deep trains reach 1e-60 and below, where the
     // products of squared norms would underflow.
     __shared__ int s_exp;
-    for (int j = tid; j < k; j += SVD_NT) {
+    for (int j = tid; j < k; j += nth) {
         double s = 0.0;
         for (int i = 0; i < m; i++) s = fmax(s, fabs(A[i + (long)j * lda]));
         s_sig[j] = s;
@@ -609,25 +610,25 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
             if (!(s_sig[j] > 0.0)) s_perm[q++] = j;
     }
     __syncthreads();
-    for (long e = tid; e < (long)m * k; e += SVD_NT) {
+    for (long e = tid; e < (long)m * k; e += nth) {
         long j = e / m, i = e - j * m;
         W[i + j * ldw] = ldexp(A[i + (long)s_perm[j] * lda], -s_exp);
     }
     if (Vw)
-        for (long e = tid; e < (long)k * k; e += SVD_NT) {
+        for (long e = tid; e < (long)k * k; e += nth) {
             long j = e / k, i = e - j * k;
             Vw[i + j * ldvw] = (i == s_perm[j]) ? 1.0 : 0.0;
         }
     if (LOGROT)
-        for (int j = tid; j < k; j += SVD_NT) posout[1024 + j] = s_perm[j];
+        for (int j = tid; j < k; j += nth) posout[1024 + j] = s_perm[j];
     __syncthreads();
     const int knz = s_knz;
 
     const int kp = knz + (knz & 1);
     const int P = kp / 2;
     int G = 64;
-    while (G > 1 && G * P > SVD_NT) G >>= 1;
-    const int ngroups = SVD_NT / G;
+    while (G > 1 && G * P > nth) G >>= 1;
+    const int ngroups = nth / G;
     const int g = tid / G, gl = tid - g * G;
     // Rotate only when the pair's cosine exceeds the rounding noise of an
     // m-term dot product; below that the columns are orthogonal to working
@@ -731,7 +732,7 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
     }
     __syncthreads();
     // descending order by rank counting (ties by index)
-    for (int j = tid; j < k; j += SVD_NT) {
+    for (int j = tid; j < k; j += nth) {
         double sj = s_sig[j];
         int pos = 0;
         for (int i = 0; i < k; i++) {
@@ -741,18 +742,18 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
         s_pos[j] = pos;
     }
     __syncthreads();
-    for (int j = tid; j < k; j += SVD_NT) {
+    for (int j = tid; j < k; j += nth) {
         sig[s_pos[j]] = ldexp(s_sig[j], s_exp);
         s_sorted[s_pos[j]] = s_sig[j];  // scaled by 2^-e
         if (LOGROT) posout[j] = s_pos[j];
     }
-    for (long e = tid; e < (long)m * k; e += SVD_NT) {
+    for (long e = tid; e < (long)m * k; e += nth) {
         long j = e / m, i = e - j * m;
         double sj = s_sig[j];
         U[i + (long)s_pos[j] * ldu] = sj > 0.0 ? W[i + j * ldw] / sj : 0.0;
     }
     if (Vw)
-        for (long e = tid; e < (long)k * k; e += SVD_NT) {
+        for (long e = tid; e < (long)k * k; e += nth) {
             long j = e / k, i = e - j * k;
             V[i + (long)s_pos[j] * ldv] = Vw[i + j * ldvw];
         }
@@ -888,12 +889,21 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *
     // count 4 m k^2 + 8 k^3; right vectors are counted by their replay)
     const double svd_flops = 4.0 * m * (double)k * k + 8.0 * (double)k * k * k;
     const double svd_bytes = 8.0 * (2.0 * m * (double)k + k);
+    // threads: each Jacobi pair gets G lanes with about 8 rows per lane, the
+    // whole workgroup holding all pairs of a round (one wave at least)
+    const int kp0 = k + (k & 1), P0 = kp0 / 2;
+    int Gw = 1;
+    while (Gw < 64 && Gw * 8 < m) Gw <<= 1;
+    int nth = P0 * Gw;
+    nth = ((nth + 63) / 64) * 64;
+    if (nth > SVD_NT) nth = SVD_NT;
+    if (nth < 64) nth = 64;
     const size_t kk = (size_t)(k | 1) * k;  // padded V footprint (LDS case)
     const size_t mk = (size_t)m * k;
     const size_t mkl = (size_t)(m | 1) * k;  // LDS footprint of the padded columns
     if (mkl + (V ? kk : 0) <= (size_t)SVD_LDS_DOUBLES) {
         PTT_LAUNCH_FB("svd_jacobi", svd_flops + (V ? 9.0 * k * (double)k * k : 0.0), svd_bytes,
-                   (k_svd_jacobi<true, true, false>), dim3(1), dim3(SVD_NT),
+                   (k_svd_jacobi<true, true, false>), dim3(1), dim3(nth),
                    (mkl + (V ? kk : 0)) * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv,
                    log_scale, delta, cap, iout, (double *)nullptr, (double2 *)nullptr,
                    (int *)nullptr);
@@ -905,7 +915,7 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *
         double2 *rotlog = (double2 *)ws;
         int *pos = (int *)(ws + logn * sizeof(double2));
         PTT_LAUNCH_FB("svd_jacobi", svd_flops, svd_bytes, (k_svd_jacobi<true, false, true>),
-                   dim3(1), dim3(SVD_NT),
+                   dim3(1), dim3(nth),
                    mkl * sizeof(double), m, k, A, lda, sig, U, ldu, V, ldv, log_scale, delta, cap,
                    iout, (double *)nullptr, rotlog, pos);
         if (V)
@@ -915,7 +925,7 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *
     } else {
         double *gwork = (double *)ptt_hip_scratch(PTT_WS_SKETCH, (mk + (V ? kk : 0)) * sizeof(double));
         PTT_LAUNCH_FB("svd_jacobi", svd_flops + (V ? 9.0 * k * (double)k * k : 0.0), svd_bytes,
-                   (k_svd_jacobi<false, false, false>), dim3(1), dim3(SVD_NT), 0, m, k,
+                   (k_svd_jacobi<false, false, false>), dim3(1), dim3(nth), 0, m, k,
                    A, lda, sig, U, ldu, V, ldv, log_scale, delta, cap, iout, gwork,
                    (double2 *)nullptr, (int *)nullptr);
     }

@@ -32,6 +32,7 @@ void attac_exp_euler_st(attac_tt *f_old, attac_tt *f_new, attac_tt *work,
 /* the reference ships only explicit Euler).                           */
 /* ------------------------------------------------------------------ */
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -55,13 +56,152 @@ static paratt *ptt_axpby_new(double a, paratt *x, double b, paratt *y) {
     return z;
 }
 
-static void round_rel_cap(paratt *x, double eps, int cap) {
+/* ---- learned rank prediction for the step truncations -------------------
+ * Every truncation site of the composite (role) keeps the spectrum record of
+ * its last rounding.  In learned mode the rank-predictor network maps, per
+ * bond, that record to a predicted rank, which sizes the sketched step of
+ * paratt_amd_truncate_rel_sketch; in logging mode the (features, new rank)
+ * pairs of plain roundings are collected as training data.
+ * Features (18, FP32): log10(sigma_{rho-13+q}/sigma_0)/16 for q = 0..15
+ * from the previous rounding of this site (-20/16 where absent), bond index
+ * i/(d-1), and the previous kept rank rho/64.  Target: the new rank / 64. */
+enum { ROLE_V, ROLE_HD, ROLE_F, ROLE_U2, ROLE_U3, ROLE_U4, ROLE_S, ROLE_UN, NROLES };
+#define LN_FEAT 18
+#define LN_SPEC 20
+static struct {
+    int valid, d;
+    double spec[64][LN_SPEC];
+} g_hist[NROLES];
+static struct {
+    int mode;
+    float *dP;  /* device parameters */
+    int nparam;
+    int log_on, nlog, caplog;
+    float *logX, *logY;
+    long n_pred, n_att, n_acc;
+} g_learn;
+
+static void make_features(const double *spec, int i, int d, float *f) {
+    const double s0 = spec[16];
+    for (int q = 0; q < 16; q++) {
+        double v = (spec[q] > 0.0 && s0 > 0.0) ? log10(spec[q] / s0) : -20.0;
+        if (v < -20.0) v = -20.0;
+        f[q] = (float)(v / 16.0);
+    }
+    f[16] = (float)((double)i / (double)(d > 1 ? d - 1 : 1));
+    f[17] = (float)(spec[17] / 64.0);
+}
+
+void paratt_amd_learned_set_params(const float *P, int nparam) {
+    if (nparam != ptt_hip_mlp_nparam()) {
+        fprintf(stderr, "paratt_amd: rank predictor expects %d parameters, got %d\n",
+                ptt_hip_mlp_nparam(), nparam);
+        exit(1);
+    }
+    if (!g_learn.dP) g_learn.dP = (float *)ptt_hip_malloc(sizeof(float) * (size_t)nparam);
+    ptt_hip_h2d(g_learn.dP, P, sizeof(float) * (size_t)nparam);
+    g_learn.nparam = nparam;
+}
+
+void paratt_amd_learned_mode(int on) {
+    if (on == 1 && !g_learn.dP) {
+        fprintf(stderr, "paratt_amd: learned mode needs paratt_amd_learned_set_params first\n");
+        exit(1);
+    }
+    g_learn.mode = on;
+}
+
+void paratt_amd_learned_reset_history(void) { memset(g_hist, 0, sizeof g_hist); }
+
+void paratt_amd_learned_log(int on, int capacity) {
+    g_learn.log_on = on;
+    if (on && capacity > g_learn.caplog) {
+        g_learn.logX = (float *)realloc(g_learn.logX, sizeof(float) * LN_FEAT * (size_t)capacity);
+        g_learn.logY = (float *)realloc(g_learn.logY, sizeof(float) * (size_t)capacity);
+        g_learn.caplog = capacity;
+    }
+    if (on) g_learn.nlog = 0;
+}
+
+int paratt_amd_learned_log_fetch(float *X, float *Y, int max) {
+    int n = g_learn.nlog < max ? g_learn.nlog : max;
+    memcpy(X, g_learn.logX, sizeof(float) * LN_FEAT * (size_t)n);
+    memcpy(Y, g_learn.logY, sizeof(float) * (size_t)n);
+    return n;
+}
+
+void paratt_amd_learned_counts(long *out) {
+    out[0] = g_learn.n_pred;
+    out[1] = g_learn.n_att;
+    out[2] = g_learn.n_acc;
+}
+
+static void round_rel_cap_role(paratt *x, double eps, int cap, int role) {
     int d = x->numDim;
     int *mr = (int *)malloc(sizeof(int) * (d + 1));
     for (int k = 0; k <= d; k++) mr[k] = (k == 0 || k == d) ? 1 : cap;
-    paratt_amd_truncate_rel(x, eps, mr);
+    const int have = (role >= 0 && g_hist[role].valid && g_hist[role].d == d && d <= 64);
+    float *feat = NULL;
+    if (have && (g_learn.mode == 1 || g_learn.log_on)) {
+        feat = (float *)malloc(sizeof(float) * LN_FEAT * (size_t)d);
+        for (int i = 1; i < d; i++) make_features(g_hist[role].spec[i], i, d, feat + LN_FEAT * (i - 1));
+    }
+    if (g_learn.mode == 2 && have) {
+        /* heuristic baseline: the previous rank as the prediction */
+        int *pred = (int *)calloc((size_t)d + 1, sizeof(int));
+        for (int i = 1; i < d; i++) pred[i] = (int)g_hist[role].spec[i][17];
+        paratt_amd_truncate_rel_sketch(x, eps, mr, pred);
+        int c2[2];
+        paratt_amd_last_sketch_counts(c2);
+        g_learn.n_pred++;
+        g_learn.n_att += c2[0];
+        g_learn.n_acc += c2[1];
+        free(pred);
+    } else if (g_learn.mode == 1 && feat) {
+        /* one forward launch for all bonds */
+        int ns = d - 1;
+        float *dX = (float *)ptt_hip_scratch(PTT_WS_COEF, sizeof(float) * (LN_FEAT + 1) * (size_t)ns + 64);
+        float *dO = dX + LN_FEAT * ns;
+        ptt_hip_h2d(dX, feat, sizeof(float) * LN_FEAT * (size_t)ns);
+        ptt_hip_mlp_forward(ns, dX, g_learn.dP, dO);
+        float *out = (float *)malloc(sizeof(float) * (size_t)ns);
+        ptt_hip_d2h(out, dO, sizeof(float) * (size_t)ns);
+        int *pred = (int *)calloc((size_t)d + 1, sizeof(int));
+        for (int i = 1; i < d; i++) {
+            long p = lrint(64.0 * out[i - 1]);
+            pred[i] = p < 1 ? 1 : (p > 1024 ? 1024 : (int)p);
+        }
+        paratt_amd_truncate_rel_sketch(x, eps, mr, pred);
+        int c2[2];
+        paratt_amd_last_sketch_counts(c2);
+        g_learn.n_pred++;
+        g_learn.n_att += c2[0];
+        g_learn.n_acc += c2[1];
+        free(pred);
+        free(out);
+    } else {
+        paratt_amd_truncate_rel(x, eps, mr);
+    }
+    if (role >= 0) {
+        double spec[64 * LN_SPEC];
+        paratt_amd_last_round_spectra(spec);
+        if (g_learn.log_on && feat) {
+            for (int i = 1; i < d && g_learn.nlog < g_learn.caplog; i++) {
+                memcpy(g_learn.logX + LN_FEAT * (size_t)g_learn.nlog, feat + LN_FEAT * (i - 1),
+                       sizeof(float) * LN_FEAT);
+                g_learn.logY[g_learn.nlog] = (float)(spec[i * LN_SPEC + 17] / 64.0);
+                g_learn.nlog++;
+            }
+        }
+        memcpy(g_hist[role].spec, spec, sizeof spec);
+        g_hist[role].valid = 1;
+        g_hist[role].d = d;
+    }
+    free(feat);
     free(mr);
 }
+
+static void round_rel_cap(paratt *x, double eps, int cap) { round_rel_cap_role(x, eps, cap, -1); }
 
 void paratt_amd_mode_sum(paratt *u, const double *sten3, paratt *s) {
     int d = u->numDim;
@@ -125,15 +265,15 @@ void paratt_amd_rxn_burgers_field(paratt *u, paratt *out, int nargs, ...) {
     paratt_amd_mode_sum(u, d2, lap);
     paratt *v0 = ptt_axpby_new(1.0, ones, -1.0, u);
     paratt *v = ptt_axpby_new(1.0, v0, -1.0, s1);
-    round_rel_cap(v, g_rb.eps, g_rb.cap);
+    round_rel_cap_role(v, g_rb.eps, g_rb.cap, ROLE_V);
     int *rh = (int *)malloc(sizeof(int) * (d + 1));
     for (int k = 0; k <= d; k++) rh[k] = u->x->ttRank[k] * v->x->ttRank[k];
     paratt *hd = alloc_ptt(d, u->dim, rh);
     pattacHadamard(d, u->dim, u->x->ttRank, u->x->coreList, v->x->ttRank, v->x->coreList,
                    hd->x->ttRank, hd->x->coreList);
-    round_rel_cap(hd, g_rb.eps, g_rb.cap);
+    round_rel_cap_role(hd, g_rb.eps, g_rb.cap, ROLE_HD);
     paratt *f = ptt_axpby_new(1.0, hd, g_rb.nu, lap);
-    round_rel_cap(f, g_rb.eps, g_rb.cap);
+    round_rel_cap_role(f, g_rb.eps, g_rb.cap, ROLE_F);
     copy_rank_change_tt(out->x, f->x);
     free_ptt(ones); free_ptt(s1); free_ptt(lap); free_ptt(v0); free_ptt(v); free_ptt(hd);
     free_ptt(f);
@@ -152,20 +292,20 @@ void paratt_amd_rk4_st(paratt *f_old, paratt *f_new,
     paratt *k3 = alloc_ptt(d, f_old->dim, one), *k4 = alloc_ptt(d, f_old->dim, one);
     vecField(f_old, k1, 0);
     paratt *u2 = ptt_axpby_new(1.0, f_old, 0.5 * dt, k1);
-    round_rel_cap(u2, eps, cap);
+    round_rel_cap_role(u2, eps, cap, ROLE_U2);
     vecField(u2, k2, 0);
     paratt *u3 = ptt_axpby_new(1.0, f_old, 0.5 * dt, k2);
-    round_rel_cap(u3, eps, cap);
+    round_rel_cap_role(u3, eps, cap, ROLE_U3);
     vecField(u3, k3, 0);
     paratt *u4 = ptt_axpby_new(1.0, f_old, dt, k3);
-    round_rel_cap(u4, eps, cap);
+    round_rel_cap_role(u4, eps, cap, ROLE_U4);
     vecField(u4, k4, 0);
     paratt *s12 = ptt_axpby_new(1.0, k1, 2.0, k2);
     paratt *s123 = ptt_axpby_new(1.0, s12, 2.0, k3);
     paratt *s = ptt_axpby_new(1.0, s123, 1.0, k4);
-    round_rel_cap(s, eps, cap);
+    round_rel_cap_role(s, eps, cap, ROLE_S);
     paratt *un = ptt_axpby_new(1.0, f_old, dt / 6.0, s);
-    round_rel_cap(un, eps, cap);
+    round_rel_cap_role(un, eps, cap, ROLE_UN);
     copy_rank_change_tt(f_new->x, un->x);
     free_ptt(k1); free_ptt(k2); free_ptt(k3); free_ptt(k4); free_ptt(u2); free_ptt(u3);
     free_ptt(u4); free_ptt(s12); free_ptt(s123); free_ptt(s); free_ptt(un);

@@ -21,6 +21,7 @@
  * core (one GEMM, no explicit Q), otherwise V^T times the explicit Q.
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -33,10 +34,30 @@
 #define KAPPA_ONE_PASS 30.0
 #define CHOLQR_MAX_PASSES 6
 #define KAPPA_TWO_PASS 1.0e6
+#define SKETCH_OVERSAMPLE 8
 
 static paratt_amd_round_stats g_stats;
 
 void paratt_amd_last_round_stats(paratt_amd_round_stats *out) { *out = g_stats; }
+
+/* Per-bond spectrum record of the last rounding (bond i = 1..d-1, i < 64):
+ * [0..15] sigma_{rho-13+j} (0 outside the spectrum), [16] sigma_0,
+ * [17] rho, [18] incoming rank, [19] 1 if the bond was set by a sketch. */
+#define SPEC_W 20
+static double g_spec[64][SPEC_W];
+
+void paratt_amd_last_round_spectra(double *out) { memcpy(out, g_spec, sizeof g_spec); }
+
+/* Sketched right-to-left steps: per-bond predicted ranks (NULL: none) and
+ * counters {attempted, accepted} of the last rounding. */
+static const int *g_pred = NULL;
+static int g_sketch_counts[2];
+static uint64_t g_sketch_seed = 0x5eed0001ULL;
+
+void paratt_amd_last_sketch_counts(int *out) {
+    out[0] = g_sketch_counts[0];
+    out[1] = g_sketch_counts[1];
+}
 
 /* Small device workspace for one step: matrices up to c x c. */
 typedef struct {
@@ -45,10 +66,10 @@ typedef struct {
     int cap;
 } step_ws;
 
-static step_ws get_ws(int c) {
+static step_ws get_ws_slot(int c, int slot) {
     step_ws w;
     size_t cc = (size_t)c * c;
-    double *base = (double *)ptt_hip_scratch(PTT_WS_SMALL, sizeof(double) * (15 * cc + 2 * (size_t)c + 128));
+    double *base = (double *)ptt_hip_scratch(slot, sizeof(double) * (15 * cc + 2 * (size_t)c + 128));
     w.G = base;
     w.R1 = w.G + cc;
     w.T1 = w.R1 + cc;
@@ -70,6 +91,8 @@ static step_ws get_ws(int c) {
     w.cap = c;
     return w;
 }
+
+static step_ws get_ws(int c) { return get_ws_slot(c, PTT_WS_SMALL); }
 
 static double tiny_pivot(void) { return 1.0e-14; }
 
@@ -102,10 +125,12 @@ static chol_out run_chol(int c, const double *G, double shift, double *R, double
     return run_chol_k(c, G, shift, R, T, w, 1);
 }
 
-/* In-place C = alpha * A * T (A: len x c, ld lda; T upper c x c).  Uses the
- * TMP slot when the GEMM may not run in place (c > 128). */
-static void apply_right(attac_tt *x, int core, long len, int c, const double *T, double alpha) {
-    double *A = x->coreList[core];
+/* In-place C = alpha * A * T (A: len x c, ld len; T c x c), A being core
+ * `core` of x or, when buf is given, that buffer (c <= 128).  Uses the TMP
+ * slot when the GEMM may not run in place (c > 128). */
+static void apply_right(attac_tt *x, int core, double *buf, long len, int c, const double *T,
+                        double alpha) {
+    double *A = buf ? buf : x->coreList[core];
     if (c <= 128) {
         ptt_hip_dgemm('N', 'N', (int)len, c, c, alpha, A, len, T, c, 0.0, A, len);
     } else {
@@ -115,9 +140,11 @@ static void apply_right(attac_tt *x, int core, long len, int c, const double *T,
     }
 }
 
-/* In-place C = alpha * T^T * M (M: c x len, ld c; T upper c x c). */
-static void apply_left_t(attac_tt *x, int core, int c, long len, const double *T, double alpha) {
-    double *M = x->coreList[core];
+/* In-place C = alpha * T^T * M (M: c x len, ld c; T c x c); M as in
+ * apply_right. */
+static void apply_left_t(attac_tt *x, int core, double *buf, int c, long len, const double *T,
+                         double alpha) {
+    double *M = buf ? buf : x->coreList[core];
     if (c <= 128) {
         ptt_hip_dgemm('T', 'N', c, (int)len, c, alpha, T, c, M, c, 0.0, M, c);
     } else {
@@ -131,14 +158,14 @@ static void apply_left_t(attac_tt *x, int core, int c, long len, const double *T
  * c x len (trans 'N': rows, RTL).  On return the core holds Q (alpha
  * applied) and R (c x c upper, A = Q*R resp. M = R^T*Q) is in *Rout.
  * Returns the number of passes. */
-static int cholqr_core(attac_tt *x, int core, char trans, long len, int c, double qalpha,
-                       step_ws *w, const double **Rout, int allow_one_pass_noq, int *one_pass,
-                       const double **T1out) {
-    double *A = x->coreList[core];
+static int cholqr_core(attac_tt *x, int core, double *buf, char trans, long len, int c,
+                       double qalpha, step_ws *w, const double **Rout, int allow_one_pass_noq,
+                       int *one_pass, const double **T1out) {
+    double *A = buf ? buf : x->coreList[core];
     ptt_hip_gram(trans, c, len, A, trans == 'T' ? len : c, w->G);
     chol_out o = run_chol(c, w->G, 0.0, w->R1, w->T1, w);
-    if (trans == 'T' && core < 64) g_stats.kappa_ltr[core] = o.kappa;
-    if (trans == 'N' && core < 64) g_stats.kappa_rtl[core] = o.kappa;
+    if (!buf && trans == 'T' && core < 64) g_stats.kappa_ltr[core] = o.kappa;
+    if (!buf && trans == 'N' && core < 64) g_stats.kappa_rtl[core] = o.kappa;
     if (one_pass) *one_pass = 0;
     if (o.deficient == 0 && o.kappa <= KAPPA_ONE_PASS) {
         if (allow_one_pass_noq) {
@@ -148,8 +175,8 @@ static int cholqr_core(attac_tt *x, int core, char trans, long len, int c, doubl
             if (one_pass) *one_pass = 1;
             return 1;
         }
-        if (trans == 'T') apply_right(x, core, len, c, w->T1, qalpha);
-        else apply_left_t(x, core, c, len, w->T1, qalpha);
+        if (trans == 'T') apply_right(x, core, buf, len, c, w->T1, qalpha);
+        else apply_left_t(x, core, buf, c, len, w->T1, qalpha);
         *Rout = w->R1;
         return 1;
     }
@@ -171,13 +198,13 @@ static int cholqr_core(attac_tt *x, int core, char trans, long len, int c, doubl
     }
     int passes = 1;
     /* pass 1 */
-    if (trans == 'T') apply_right(x, core, len, c, w->T1, 1.0);
-    else apply_left_t(x, core, c, len, w->T1, 1.0);
+    if (trans == 'T') apply_right(x, core, buf, len, c, w->T1, 1.0);
+    else apply_left_t(x, core, buf, c, len, w->T1, 1.0);
     double *Racc = w->Racc;
     ptt_hip_d2d(Racc, w->R1, sizeof(double) * (size_t)c * c);
     while (need > 0) {
         passes++;
-        A = x->coreList[core];
+        A = buf ? buf : x->coreList[core];
         ptt_hip_gram(trans, c, len, A, trans == 'T' ? len : c, w->G);
         double *Rp = (passes % 2 == 0) ? w->R2 : w->R3;
         double *Tp = (passes % 2 == 0) ? w->T2 : w->T3;
@@ -190,8 +217,8 @@ static int cholqr_core(attac_tt *x, int core, char trans, long len, int c, doubl
         }
         if (passes >= CHOLQR_MAX_PASSES) need = 0;
         const double a = (need == 0) ? qalpha : 1.0;
-        if (trans == 'T') apply_right(x, core, len, c, Tp, a);
-        else apply_left_t(x, core, c, len, Tp, a);
+        if (trans == 'T') apply_right(x, core, buf, len, c, Tp, a);
+        else apply_left_t(x, core, buf, c, len, Tp, a);
         /* Racc <- Rp * Racc */
         ptt_hip_trmm_small(c, Rp, Racc, w->Rtmp);
         ptt_hip_d2d(Racc, w->Rtmp, sizeof(double) * (size_t)c * c);
@@ -229,7 +256,7 @@ static void ltr_sweep(attac_tt *x, const double *qscale, const double *rscale) {
         }
         step_ws w = get_ws(c);
         const double *R = NULL;
-        int passes = cholqr_core(x, i, 'T', m, c, qscale[i], &w, &R, 0, NULL, NULL);
+        int passes = cholqr_core(x, i, NULL, 'T', m, c, qscale[i], &w, &R, 0, NULL, NULL);
         if (i < 64) g_stats.ltr_cholqr_passes[i] = passes;
         /* core i+1 (c x wcols) <- rscale * R * core i+1 */
         if (c <= 128) {
@@ -267,7 +294,7 @@ static int rtl_step_robust(attac_tt *x, int i, const int *maxr, double delta,
     long w = (long)n[i] * r[i + 1];
     const double *R = NULL, *T1 = NULL;
     int one_pass = 0;
-    int passes = cholqr_core(x, i, 'N', w, rows, 1.0, ws, &R, 0, &one_pass, &T1);
+    int passes = cholqr_core(x, i, NULL, 'N', w, rows, 1.0, ws, &R, 0, &one_pass, &T1);
     *passes_out = passes;
     ptt_hip_transpose(rows, rows, R, rows, ws->L, rows, 1.0);
     ptt_hip_svd_jacobi(rows, rows, ws->L, rows, ws->sig, ws->U, rows, ws->V, rows, 0.0, delta,
@@ -365,6 +392,82 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
     return rho;
 }
 
+/* Sketched right-to-left step: a randomized range finder (Halko, Martinsson
+ * and Tropp, SIAM Review 2011, Alg. 4.4: Gaussian test matrix, one power
+ * iteration) with l = predicted rank + oversampling columns replaces the
+ * rows x rows factorization and SVD of the plain step by l x l ones.
+ *
+ * Certification (the rank must equal the plain method's): with Q the
+ * orthonormal sketch basis, B = Q^T M and sigma^_j its singular values,
+ * sigma^_j <= sigma_j (interlacing), and the exact residual energy
+ * e2 = ||M - Q B||^2 satisfies e2 + sum_{q>=j, q<l} sigma^_q^2
+ * >= sum_{q>=j} sigma_q^2 = tail_j^2.  The candidate rank rho (the tail rule
+ * on the sketch with e2 added) thus has tail_rho < delta, so the plain rank
+ * is <= rho; and if sigma^_{rho-1} >= delta then tail_{rho-1} >= delta, so
+ * the plain rank is >= rho.  Anything else -- the residual alone above
+ * delta, the lower bound failing, or the cap binding (where the truncation
+ * error is not eps-controlled) -- returns -1 and the plain step runs on the
+ * untouched core.  On acceptance the new core is V_rho^T Q_B (orthonormal
+ * rows) and ws->US = Q U_rho Sigma_rho, as in the plain steps. */
+static int rtl_step_sketch(attac_tt *x, int i, const int *maxr, double delta,
+                           const double *nscale, step_ws *ws, int l) {
+    int *r = x->ttRank, *n = x->dim;
+    const int rows = r[i];
+    const long w = (long)n[i] * r[i + 1];
+    double *M = x->coreList[i];
+    const size_t need = (size_t)w * l + 2 * (size_t)rows * l + (size_t)l * w +
+                        (size_t)rows * w + (size_t)l * l;
+    double *Om = (double *)ptt_hip_scratch(PTT_WS_RSK, sizeof(double) * need);
+    double *Y = Om + (size_t)w * l;       /* rows x l: sketch, then basis Q */
+    double *QL = Y + (size_t)rows * l;    /* rows x l */
+    double *B = QL + (size_t)rows * l;    /* l x w */
+    double *E = B + (size_t)l * w;        /* rows x w: residual */
+    double *USB = E + (size_t)rows * w;   /* l x l */
+    step_ws sw = get_ws_slot(l, PTT_WS_SMALL2);
+    const double *R = NULL;
+    ptt_hip_gauss_fill(Om, w * l, g_sketch_seed++, 0, 1.0);
+    ptt_hip_dgemm('N', 'N', rows, l, (int)w, 1.0, M, rows, Om, w, 0.0, Y, rows);
+    cholqr_core(NULL, 0, Y, 'T', rows, l, 1.0, &sw, &R, 0, NULL, NULL);
+    /* one power iteration: Y <- M (M^T Q) */
+    ptt_hip_dgemm('T', 'N', (int)w, l, rows, 1.0, M, rows, Y, rows, 0.0, Om, w);
+    ptt_hip_dgemm('N', 'N', rows, l, (int)w, 1.0, M, rows, Om, w, 0.0, Y, rows);
+    cholqr_core(NULL, 0, Y, 'T', rows, l, 1.0, &sw, &R, 0, NULL, NULL);
+    /* B = Q^T M = L_B Q_B (LQ by Cholesky-QR of the rows) */
+    ptt_hip_dgemm('T', 'N', l, (int)w, rows, 1.0, Y, rows, M, rows, 0.0, B, l);
+    cholqr_core(NULL, 0, B, 'N', w, l, 1.0, &sw, &R, 0, NULL, NULL);
+    ptt_hip_transpose(l, l, R, l, sw.L, l, 1.0);
+    /* residual of the rank-l approximation Q L_B Q_B, exact energy */
+    ptt_hip_dgemm('N', 'N', rows, l, l, 1.0, Y, rows, sw.L, l, 0.0, QL, rows);
+    ptt_hip_d2d(E, M, sizeof(double) * (size_t)rows * w);
+    ptt_hip_dgemm('N', 'N', rows, (int)w, l, -1.0, QL, rows, B, l, 1.0, E, rows);
+    ptt_hip_sumsq(rows, (int)w, E, rows, sw.dinfo + 4);
+    ptt_hip_svd_jacobi(l, l, sw.L, l, sw.sig, sw.U, l, sw.V, l, 0.0, delta, l, sw.iinfo + 1);
+    double *hb = (double *)malloc(sizeof(double) * ((size_t)l + 8));
+    ptt_hip_d2h(hb, sw.sig, sizeof(double) * ((size_t)l + 8));
+    const double e2 = hb[l + 4];
+    int rho = -1;
+    if (sqrt(e2) < delta) {
+        int rs = tail_rule(hb, l, e2, delta);
+        const int cap = maxr[i] < rows ? maxr[i] : rows;
+        int ok = (rs <= cap) && (rs == 0 || hb[rs - 1] >= delta);
+        if (ok) rho = rs < 1 ? 1 : rs;
+    }
+    if (rho > 0) {
+        int h_rho = rho;
+        ptt_hip_h2d(sw.iinfo + 8, &h_rho, sizeof(int));
+        /* new core (rho x w) = sqrt(n_i) V_rho^T Q_B */
+        double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)rho * w);
+        ptt_hip_dgemm('T', 'N', rho, (int)w, l, nscale[i], sw.V, l, B, l, 0.0, tmp, rho);
+        ptt_host_swap_core_with_tmp(x, i, (long)rho * w);
+        /* U Sigma of the bond: Q (U_B Sigma)_rho */
+        ptt_hip_svd_factors(l, rho, sw.iinfo + 8, sw.U, l, sw.sig, 1.0, NULL, USB);
+        ptt_hip_dgemm('N', 'N', rows, rho, l, 1.0, Y, rows, USB, l, 0.0, ws->US, rows);
+        ptt_hip_d2d(ws->sig, sw.sig, sizeof(double) * (size_t)l);
+    }
+    free(hb);
+    return rho;
+}
+
 /* Right-to-left truncation (pattac.c:297-522) of a left-orthogonal TT whose
  * cores carry the reference's scaled magnitudes; tol is absolute in that
  * scaled space.  nscale[k] = sqrt(n_k) is applied to each finished core. */
@@ -396,11 +499,21 @@ static void rtl_sweep(attac_tt *x, const int *maxr, double abs_tol, const double
             ptt_hip_svd_factors(rows, rho, ws.iinfo + 2, ws.U, rows, ws.sig, 1.0, NULL, ws.US);
         } else {
             ws = get_ws(rows);
+            int sk = -1;
+            if (g_pred && i < 64 && g_pred[i] > 0) {
+                const int l = g_pred[i] + SKETCH_OVERSAMPLE;
+                if (l <= 128 && 2 * l <= rows && (long)l <= w) {
+                    g_sketch_counts[0]++;
+                    sk = rtl_step_sketch(x, i, maxr, delta, nscale, &ws, l);
+                    if (sk > 0) g_sketch_counts[1]++;
+                }
+            }
             /* Bond spectra change slowly along the train: once the Gram
              * route was rejected at the right neighbour, go straight to the
              * orthogonal route (saves a Gram, a factorization and an SVD). */
-            rho = prev_robust ? -1 : rtl_step_fast(x, i, maxr, delta, nscale, &ws);
-            if (i < 64) g_stats.rtl_cholqr_passes[i] = 1;
+            rho = (sk > 0) ? sk : prev_robust ? -1 : rtl_step_fast(x, i, maxr, delta, nscale, &ws);
+            if (i < 64) g_stats.rtl_cholqr_passes[i] = (sk > 0) ? 20 : 1;
+            if (i < 64) g_spec[i][19] = (sk > 0) ? 1.0 : 0.0;
             prev_robust = (rho < 0);
             if (rho < 0) {
                 int passes = 0;
@@ -411,12 +524,26 @@ static void rtl_sweep(attac_tt *x, const int *maxr, double abs_tol, const double
             }
         }
         if (i < 64) {
-            double sv[4] = {0, 0, 0, 0};
+            /* the leading spectrum up to rho+3 in one transfer: the values
+             * around the cut for the statistics and the spectrum record */
+            double sv[132];
             int k = (int)(w < rows ? w : rows);
-            int lo = rho - 1, cnt = 0;
-            for (int q = lo; q < lo + 4 && q < k; q++) cnt++;
-            if (cnt > 0) ptt_hip_d2h(sv, ws.sig + lo, sizeof(double) * cnt);
-            for (int q = 0; q < 4; q++) g_stats.sig[i][q] = sv[q];
+            /* sketch: the l Ritz values (lower bounds of the spectrum) */
+            if (g_spec[i][19] != 0.0 && g_pred) k = g_pred[i] + SKETCH_OVERSAMPLE;
+            int cnt = rho + 3 < k ? rho + 3 : k;
+            if (cnt > 128) cnt = 128;
+            if (cnt > 0) ptt_hip_d2h(sv, ws.sig, sizeof(double) * cnt);
+            for (int q = 0; q < 4; q++) {
+                int j = rho - 1 + q;
+                g_stats.sig[i][q] = (j >= 0 && j < cnt) ? sv[j] : 0.0;
+            }
+            for (int q = 0; q < 16; q++) {
+                int j = rho - 13 + q;
+                g_spec[i][q] = (j >= 0 && j < cnt) ? sv[j] : 0.0;
+            }
+            g_spec[i][16] = cnt > 0 ? sv[0] : 0.0;
+            g_spec[i][17] = rho;
+            g_spec[i][18] = rows;
         }
         /* core i-1 (m1 x rows) <- left_alpha * core i-1 * U_rho Sigma_rho */
         if (rho <= 128) {
@@ -444,6 +571,8 @@ static void fill_scales(const attac_tt *x, double *inv, double *sq) {
 static double round_impl(attac_tt *x, const int *maxRank, double tol, int tol_is_rel) {
     int d = x->numDim;
     memset(&g_stats, 0, sizeof g_stats);
+    memset(g_spec, 0, sizeof g_spec);
+    g_sketch_counts[0] = g_sketch_counts[1] = 0;
     g_stats.d = d;
     int *mr = (int *)malloc(sizeof(int) * (d + 1));
     memcpy(mr, maxRank, sizeof(int) * (d + 1)); /* maxRank may alias ttRank */
@@ -491,6 +620,13 @@ void paratt_truncate(paratt *x_ptt, double tol, int *maxRank) {
 
 void paratt_amd_truncate_rel(paratt *x_ptt, double eps_rel, int *maxRank) {
     round_impl(x_ptt->x, maxRank, eps_rel, 1);
+}
+
+void paratt_amd_truncate_rel_sketch(paratt *x_ptt, double eps_rel, int *maxRank,
+                                    const int *pred_rank) {
+    g_pred = pred_rank;
+    round_impl(x_ptt->x, maxRank, eps_rel, 1);
+    g_pred = NULL;
 }
 
 void pattacTruncate(MPI_Comm comm, const int d, int *distrib_dim_mat, int *rx, double **x,
