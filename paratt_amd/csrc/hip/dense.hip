@@ -24,7 +24,7 @@ constexpr int LDS_DOUBLES = 17408;  // 136 KB dynamic + static arrays <= 160 KB 
 // The Jacobi kernel holds 24 KB of static LDS (sorting and packing tables),
 // so its dynamic budget is smaller: 17280 doubles + 24.1 KB < 160 KB.
 constexpr int SVD_LDS_DOUBLES = 17280;
-constexpr int SVD_NT = 1024;  // Jacobi SVD workgroup: 16 waves, four per SIMD
+constexpr int SVD_NT = 1024;  // Jacobi SVD workgroup: at most 16 waves, four per SIMD
 constexpr int SVD_RV = 8;     // rows per lane held in registers by a Jacobi pair
 
 // Work matrix S is stored ROW-major in LDS (S[i*c + k] = entry (i,k)) so the
@@ -234,6 +234,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
                                                   int *info, double *dinfo, int want_kappa) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
+    const int nth = blockDim.x;  // 64..1024, a multiple of 64, chosen for c
     double *S = lds;              // c x c row-major
     double *D = S + (long)c * c;  // Jacobi scaling
     double *tdg = D + c;          // raw pivots, then the diagonal of Ts
@@ -256,19 +257,19 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         __syncthreads();
         a = 0.0;
         b = 0.0;
-        for (int i = 0; i < NT / 64; i++) {
+        for (int i = 0; i < nth / 64; i++) {
             a += s_red[2 * i];
             b += s_red[2 * i + 1];
         }
     };
 
-    for (int j = tid; j < c; j += NT) {
+    for (int j = tid; j < c; j += nth) {
         double g = G[j + (long)j * c];
         D[j] = g > 0.0 ? sqrt(g) : 0.0;
     }
     if (tid == 0) s_def = 0;
     __syncthreads();
-    for (long e = tid; e < (long)c * c; e += NT) {
+    for (long e = tid; e < (long)c * c; e += nth) {
         int k = (int)(e % c), i = (int)(e / c);
         double di = D[i], dk = D[k];
         S[e] = (di > 0.0 && dk > 0.0) ? G[k + (long)i * c] / (di * dk) : 0.0;
@@ -277,14 +278,14 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     double nrm0 = 0.0;
     for (int i = 0; i < c; i++) nrm0 += (1.0 + 0.01 * (i % 7)) * (1.0 + 0.01 * (i % 7));
     nrm0 = 1.0 / sqrt(nrm0);
-    for (int i = tid; i < c; i += NT) pv[i] = (1.0 + 0.01 * (i % 7)) * nrm0;
+    for (int i = tid; i < c; i += nth) pv[i] = (1.0 + 0.01 * (i % 7)) * nrm0;
     __syncthreads();
     const int PIT = want_kappa ? 6 : 0;
     const int row8 = tid >> 3, sub8 = tid & 7;
     double lmax = 0.0;
     for (int it = 0; it < PIT; it++) {
         // y = S x with x of unit length: lambda ~ ||y||
-        for (int i0 = 0; i0 < c; i0 += NT / 8) {
+        for (int i0 = 0; i0 < c; i0 += nth / 8) {
             int i = i0 + row8;
             double acc = 0.0;
             if (i < c)
@@ -296,11 +297,11 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         }
         __syncthreads();
         double nn = 0.0, dummy = 0.0;
-        for (int i = tid; i < c; i += NT) nn += pv[c + i] * pv[c + i];
+        for (int i = tid; i < c; i += nth) nn += pv[c + i] * pv[c + i];
         block_sum2(nn, dummy);
         lmax = sqrt(nn);
         double inv = nn > 0.0 ? 1.0 / lmax : 0.0;
-        for (int i = tid; i < c; i += NT) pv[i] = pv[c + i] * inv;
+        for (int i = tid; i < c; i += nth) pv[i] = pv[c + i] * inv;
         __syncthreads();
     }
     // Cholesky, upper triangle of S, one barrier per step.
@@ -312,7 +313,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
             const double pq = tdg[q];
             const double rqq = pq > 0.0 ? sqrt(pq) : 0.0;
             const double iq = pq > 0.0 ? 1.0 / rqq : 0.0;
-            for (int k = q + tid; k < c; k += NT) S[(long)q * c + k] = (k == q) ? rqq : S[(long)q * c + k] * iq;
+            for (int k = q + tid; k < c; k += nth) S[(long)q * c + k] = (k == q) ? rqq : S[(long)q * c + k] * iq;
         }
         const double p = S[(long)j * c + j] + shift;
         const bool dead = !(D[j] > 0.0) || !(p > tiny);
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         }
         if (!dead) {
             const double ip = 1.0 / p;
-            for (int i = j + 1 + hw; i < c; i += NT / 32) {
+            for (int i = j + 1 + hw; i < c; i += nth / 32) {
                 const double sji = S[(long)j * c + i] * ip;
                 for (int k = i + hl; k < c; k += 32) S[(long)i * c + k] -= sji * S[(long)j * c + k];
             }
@@ -340,32 +341,34 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     // Blocked inverse Ts = Rs^-1 (dead rows and columns zero).  Ts(i,k) for
     // i < k lives at S[k*c + i]; its diagonal in tdg.
     const int nb = (c + CHOL_BLK - 1) / CHOL_BLK;
-    for (int j = tid; j < c; j += NT) tdg[j] = (dd[j] != 0.0) ? 0.0 : 1.0 / S[(long)j * c + j];
+    for (int j = tid; j < c; j += nth) tdg[j] = (dd[j] != 0.0) ? 0.0 : 1.0 / S[(long)j * c + j];
     __syncthreads();
     auto tget = [&](int i, int k) -> double {  // Ts(i,k), i <= k
         return (i == k) ? tdg[i] : S[(long)k * c + i];
     };
     // diagonal blocks, all in parallel: column jj of every block per step
-    for (int jj = 1; jj < CHOL_BLK; jj++) {
+    for (int jj = 1; jj < CHOL_BLK && jj < c; jj++) {
         // rows i of block q with i < j = q*32 + jj; 8 lanes per (q, i)
-        const int q = row8 >> 5, il = row8 & 31;  // row8 in [0, 128)
-        const int j = q * CHOL_BLK + jj;
-        const int i = q * CHOL_BLK + il;
-        double acc = 0.0;
-        const bool act = (q < nb) && (j < c) && (il < jj);
-        if (act && dd[j] == 0.0)
-            for (int l = i + sub8; l < j; l += 8) acc += tget(i, l) * S[(long)l * c + j];
-        acc += __shfl_xor(acc, 1);
-        acc += __shfl_xor(acc, 2);
-        acc += __shfl_xor(acc, 4);
-        // writes go to row j of the lower triangle; this step reads rows < j
-        if (act && sub8 == 0) S[(long)j * c + i] = -acc * tdg[j];
+        for (int rr = row8; rr < nb * CHOL_BLK; rr += nth / 8) {
+            const int q = rr >> 5, il = rr & 31;
+            const int j = q * CHOL_BLK + jj;
+            const int i = q * CHOL_BLK + il;
+            double acc = 0.0;
+            const bool act = (j < c) && (il < jj);
+            if (act && dd[j] == 0.0)
+                for (int l = i + sub8; l < j; l += 8) acc += tget(i, l) * S[(long)l * c + j];
+            acc += __shfl_xor(acc, 1);
+            acc += __shfl_xor(acc, 2);
+            acc += __shfl_xor(acc, 4);
+            // writes go to row j of the lower triangle; this step reads rows < j
+            if (act && sub8 == 0) S[(long)j * c + i] = -acc * tdg[j];
+        }
         __syncthreads();
     }
     // off-diagonal blocks by distance: Ts_ij = -Ts_ii * sum_{l=i+1..j} Rs_il Ts_lj
     for (int sd = 1; sd < nb; sd++) {
         const int npair = nb - sd;
-        for (int e = tid; e < npair * CHOL_BLK * CHOL_BLK; e += NT) {
+        for (int e = tid; e < npair * CHOL_BLK * CHOL_BLK; e += nth) {
             const int pi = e / (CHOL_BLK * CHOL_BLK), r = e - pi * CHOL_BLK * CHOL_BLK;
             const int a = r / CHOL_BLK, b = r - a * CHOL_BLK;
             const int bi = pi, bj = pi + sd;
@@ -378,7 +381,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
             Y[e] = acc;
         }
         __syncthreads();
-        for (int e = tid; e < npair * CHOL_BLK * CHOL_BLK; e += NT) {
+        for (int e = tid; e < npair * CHOL_BLK * CHOL_BLK; e += nth) {
             const int pi = e / (CHOL_BLK * CHOL_BLK), r = e - pi * CHOL_BLK * CHOL_BLK;
             const int a = r / CHOL_BLK, b = r - a * CHOL_BLK;
             const int bi = pi, bj = pi + sd;
@@ -395,7 +398,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     }
     // outputs and Frobenius norms
     double fr = 0.0, ft = 0.0;
-    for (long e = tid; e < (long)c * c; e += NT) {
+    for (long e = tid; e < (long)c * c; e += nth) {
         int i = (int)(e % c), k = (int)(e / c);
         double rv = (i <= k) ? S[(long)i * c + k] : 0.0;
         double tv = (i <= k) ? tget(i, k) : 0.0;
@@ -406,11 +409,11 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     }
     block_sum2(fr, ft);
     // lambda_max(Ts Ts^T) by power iteration
-    for (int i = tid; i < c; i += NT) pv[i] = (1.0 + 0.01 * (i % 7)) * nrm0;
+    for (int i = tid; i < c; i += nth) pv[i] = (1.0 + 0.01 * (i % 7)) * nrm0;
     __syncthreads();
     double lminv = 0.0;
     for (int it = 0; it < PIT; it++) {
-        for (int k0 = 0; k0 < c; k0 += NT / 8) {  // u = Ts^T x
+        for (int k0 = 0; k0 < c; k0 += nth / 8) {  // u = Ts^T x
             int k = k0 + row8;
             double acc = 0.0;
             if (k < c)
@@ -421,7 +424,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
             if (k < c && sub8 == 0) pv[c + k] = acc;
         }
         __syncthreads();
-        for (int i0 = 0; i0 < c; i0 += NT / 8) {  // y = Ts u  (into Y scratch)
+        for (int i0 = 0; i0 < c; i0 += nth / 8) {  // y = Ts u  (into Y scratch)
             int i = i0 + row8;
             double acc = 0.0;
             if (i < c)
@@ -433,11 +436,11 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         }
         __syncthreads();
         double nn = 0.0, dummy = 0.0;
-        for (int i = tid; i < c; i += NT) nn += Y[i] * Y[i];
+        for (int i = tid; i < c; i += nth) nn += Y[i] * Y[i];
         block_sum2(nn, dummy);
         lminv = sqrt(nn);
         double inv = nn > 0.0 ? 1.0 / lminv : 0.0;
-        for (int i = tid; i < c; i += NT) pv[i] = Y[i] * inv;
+        for (int i = tid; i < c; i += nth) pv[i] = Y[i] * inv;
         __syncthreads();
     }
     if (tid == 0) {
@@ -862,8 +865,10 @@ void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny, d
     size_t need = (size_t)c * c + 5 * (size_t)c;
     if (c <= CHOL_FAST_C) {
         const size_t fneed = need + 3 * CHOL_BLK * CHOL_BLK;
+        // fewer threads for small factors: the steps are barrier-bound
+        const int nth = (c <= 16) ? 64 : (c <= 48) ? 256 : NT;
         PTT_LAUNCH_FB("chol_qr_factor", 2.0 * c * (double)c * c / 3.0, 8.0 * 3.0 * c * (double)c,
-                   k_chol_fast, dim3(1), dim3(NT), fneed * sizeof(double), c,
+                   k_chol_fast, dim3(1), dim3(nth), fneed * sizeof(double), c,
                    G, shift, tiny, R, T, info, dinfo, want_kappa);
     } else if (need <= (size_t)LDS_DOUBLES) {
         PTT_LAUNCH("chol_qr_factor", k_chol_qr_factor<true>, dim3(1), dim3(NT),
@@ -889,11 +894,11 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig, double *
     // count 4 m k^2 + 8 k^3; right vectors are counted by their replay)
     const double svd_flops = 4.0 * m * (double)k * k + 8.0 * (double)k * k * k;
     const double svd_bytes = 8.0 * (2.0 * m * (double)k + k);
-    // threads: each Jacobi pair gets G lanes with about 8 rows per lane, the
+    // threads: each Jacobi pair gets G lanes with about SVD_RV rows per lane, the
     // whole workgroup holding all pairs of a round (one wave at least)
     const int kp0 = k + (k & 1), P0 = kp0 / 2;
     int Gw = 1;
-    while (Gw < 64 && Gw * 8 < m) Gw <<= 1;
+    while (Gw < 64 && Gw * SVD_RV < m) Gw <<= 1;
     int nth = P0 * Gw;
     nth = ((nth + 63) / 64) * 64;
     if (nth > SVD_NT) nth = SVD_NT;

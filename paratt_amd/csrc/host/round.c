@@ -125,6 +125,13 @@ static chol_out run_chol(int c, const double *G, double shift, double *R, double
     return run_chol_k(c, G, shift, R, T, w, 1);
 }
 
+/* Shifted factorization whose diagnostics are not needed: launched without
+ * the condition estimate and without waiting for it (no host round trip). */
+static void launch_chol_shifted(int c, const double *G, double shift, double *R, double *T,
+                                step_ws *w) {
+    ptt_hip_chol_qr_factor(c, G, shift, 0.0, R, T, w->iinfo, w->dinfo, 0);
+}
+
 /* In-place C = alpha * A * T (A: len x c, ld len; T c x c), A being core
  * `core` of x or, when buf is given, that buffer (c <= 128).  Uses the TMP
  * slot when the GEMM may not run in place (c > 128). */
@@ -191,7 +198,7 @@ static int cholqr_core(attac_tt *x, int core, double *buf, char trans, long len,
     const double shift = cholqr_shift(len, c);
     int need;
     if (!(o.deficient == 0 && o.kappa <= KAPPA_TWO_PASS)) {
-        o = run_chol(c, w->G, shift, w->R1, w->T1, w);
+        launch_chol_shifted(c, w->G, shift, w->R1, w->T1, w);
         need = 2;
     } else {
         need = 1;
@@ -210,7 +217,7 @@ static int cholqr_core(attac_tt *x, int core, double *buf, char trans, long len,
         double *Tp = (passes % 2 == 0) ? w->T2 : w->T3;
         chol_out q = run_chol(c, w->G, 0.0, Rp, Tp, w);
         if (q.deficient != 0 || q.kappa > KAPPA_TWO_PASS) {
-            run_chol(c, w->G, shift, Rp, Tp, w);
+            launch_chol_shifted(c, w->G, shift, Rp, Tp, w);
             need = 2;
         } else {
             need = (q.kappa <= KAPPA_ONE_PASS) ? 0 : 1;
