@@ -126,6 +126,10 @@ def uniform_ranks(d, rank):
 # ---------------------------------------------------------------- oracle ops
 def axpby(a, x, b, y):
     d = x.d
+    if d < 2:
+        # the formal sum needs two boundary cores (attac.c:81-112 writes
+        # [x | y] into a one-core train's single n-element core)
+        raise ValueError("formal axpby needs at least two modes")
     rz = [1] + [x.r[k] + y.r[k] for k in range(1, d)] + [1]
     z = HostTT(x.n, rz, [np.zeros(rz[k] * x.n[k] * rz[k + 1]) for k in range(d)])
     rzb = _ip(rz)
@@ -207,6 +211,10 @@ def fro(x):
 
 def rel_diff(a, b):
     """||a - b||_F / ||b||_F via the formal difference TT (no cancellation)."""
+    if a.d == 1:
+        dn = float(np.linalg.norm(a.core(0).ravel() - b.core(0).ravel()) / np.sqrt(a.n[0]))
+        bn = norm(b)
+        return dn / bn if bn > 0 else dn
     dn = norm(axpby(1.0, a, -1.0, b))
     bn = norm(b)
     return dn / bn if bn > 0 else dn

@@ -122,3 +122,43 @@ def test_round_rank_deficient_nongraded_bond():
     out = to_host(dz)
     assert out.r == ref.r
     assert O.rel_diff(out, ref) < 1e-12 + eps
+
+
+@pytest.mark.parametrize("case", ["d1", "d2", "rank1", "cap1", "eps0", "n1_mode", "tiny_scale"])
+def test_round_edge_cases_match_oracle(case):
+    """Degenerate shapes and tolerances against the oracle: one core, two
+    cores, all ranks 1, every cap 1, zero tolerance, a mode of size 1, and a
+    train whose entries are ~1e-150 (scale invariance of the rank rule)."""
+    rng_seed = {"d1": 1, "d2": 2, "rank1": 3, "cap1": 4, "eps0": 5, "n1_mode": 6,
+                "tiny_scale": 7}[case]
+    if case == "d1":
+        n, r = [17], [1, 1]
+    elif case == "d2":
+        n, r = [9, 11], [1, 6, 1]
+    elif case == "rank1":
+        n, r = [5, 6, 7, 8], [1, 1, 1, 1, 1]
+    elif case == "n1_mode":
+        n, r = [6, 1, 7, 5], [1, 4, 4, 3, 1]
+    else:
+        n, r = [8, 9, 10, 7], [1, 5, 6, 4, 1]
+    x = O.gauss_tt(n, r, 100 + rng_seed)
+    if len(n) == 1:
+        z = x
+    else:
+        y = O.gauss_tt(n, r, 200 + rng_seed)
+        z = O.axpby(1.0, x, 1e-5, y)
+    if case == "tiny_scale":
+        z = O.HostTT(z.n, z.r, [c * 1e-30 for c in z.cores])
+    maxr = list(z.r)
+    eps = 1e-9
+    if case == "cap1":
+        maxr = [1] * (len(n) + 1)
+    if case == "eps0":
+        eps = 0.0
+    tol = eps * O.norm(z)
+    ref = O.truncate(z, tol, maxr)
+    dz = P.DevTT.from_host(z)
+    P.truncate(dz, tol, maxr)
+    out = to_host(dz)
+    assert out.r == ref.r, (out.r, ref.r)
+    assert O.rel_diff(out, ref) < 1e-12 + eps
