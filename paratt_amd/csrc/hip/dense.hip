@@ -655,7 +655,24 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
                 double xa[SVD_RV], xb[SVD_RV];
                 double al = 0.0, be = 0.0, ga = 0.0;
                 double *ca = W + (long)(valid ? a : 0) * ldw, *cb = W + (long)(valid ? b : 0) * ldw;
-                if (inreg) {
+                // exactly SVD_RV rows per lane (the common m = 8 G case): no
+                // per-row guards, one branch on the pair's validity
+                const bool full = (m == SVD_RV * G);
+                if (full) {
+                    if (valid) {
+#pragma unroll
+                        for (int q = 0; q < SVD_RV; q++) {
+                            xa[q] = ca[gl + q * G];
+                            xb[q] = cb[gl + q * G];
+                        }
+#pragma unroll
+                        for (int q = 0; q < SVD_RV; q++) {
+                            al += xa[q] * xa[q];
+                            be += xb[q] * xb[q];
+                            ga += xa[q] * xb[q];
+                        }
+                    }
+                } else if (inreg) {
 #pragma unroll
                     for (int q = 0; q < SVD_RV; q++) {
                         const int i = gl + q * G;
@@ -691,7 +708,13 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
                 if (LOGROT && p < P && gl == 0)
                     rotlog[((long)sweep * (kp - 1) + t) * P + p] = make_double2(cs, sn);
                 if (rot) {
-                    if (inreg) {
+                    if (full) {
+#pragma unroll
+                        for (int q = 0; q < SVD_RV; q++) {
+                            ca[gl + q * G] = cs * xa[q] - sn * xb[q];
+                            cb[gl + q * G] = sn * xa[q] + cs * xb[q];
+                        }
+                    } else if (inreg) {
 #pragma unroll
                         for (int q = 0; q < SVD_RV; q++) {
                             const int i = gl + q * G;

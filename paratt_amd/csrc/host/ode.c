@@ -47,7 +47,7 @@ static paratt *ptt_clone(paratt *s) {
 
 static paratt *ptt_axpby_new(double a, paratt *x, double b, paratt *y) {
     int d = x->numDim;
-    int *rz = (int *)malloc(sizeof(int) * (d + 1));
+    int *rz = (int *)ptt_host_malloc(sizeof(int) * (d + 1));
     for (int k = 0; k <= d; k++)
         rz[k] = (k == 0 || k == d) ? 1 : x->x->ttRank[k] + y->x->ttRank[k];
     paratt *z = alloc_ptt(d, x->dim, rz);
@@ -116,8 +116,8 @@ void paratt_amd_learned_reset_history(void) { memset(g_hist, 0, sizeof g_hist); 
 void paratt_amd_learned_log(int on, int capacity) {
     g_learn.log_on = on;
     if (on && capacity > g_learn.caplog) {
-        g_learn.logX = (float *)realloc(g_learn.logX, sizeof(float) * LN_FEAT * (size_t)capacity);
-        g_learn.logY = (float *)realloc(g_learn.logY, sizeof(float) * (size_t)capacity);
+        g_learn.logX = (float *)ptt_host_realloc(g_learn.logX, sizeof(float) * LN_FEAT * (size_t)capacity);
+        g_learn.logY = (float *)ptt_host_realloc(g_learn.logY, sizeof(float) * (size_t)capacity);
         g_learn.caplog = capacity;
     }
     if (on) g_learn.nlog = 0;
@@ -138,17 +138,17 @@ void paratt_amd_learned_counts(long *out) {
 
 static void round_rel_cap_role(paratt *x, double eps, int cap, int role) {
     int d = x->numDim;
-    int *mr = (int *)malloc(sizeof(int) * (d + 1));
+    int *mr = (int *)ptt_host_malloc(sizeof(int) * (d + 1));
     for (int k = 0; k <= d; k++) mr[k] = (k == 0 || k == d) ? 1 : cap;
     const int have = (role >= 0 && g_hist[role].valid && g_hist[role].d == d && d <= 64);
     float *feat = NULL;
     if (have && (g_learn.mode == 1 || g_learn.log_on)) {
-        feat = (float *)malloc(sizeof(float) * LN_FEAT * (size_t)d);
+        feat = (float *)ptt_host_malloc(sizeof(float) * LN_FEAT * (size_t)d);
         for (int i = 1; i < d; i++) make_features(g_hist[role].spec[i], i, d, feat + LN_FEAT * (i - 1));
     }
     if (g_learn.mode == 2 && have) {
         /* heuristic baseline: the previous rank as the prediction */
-        int *pred = (int *)calloc((size_t)d + 1, sizeof(int));
+        int *pred = (int *)ptt_host_calloc((size_t)d + 1, sizeof(int));
         for (int i = 1; i < d; i++) pred[i] = (int)g_hist[role].spec[i][17];
         paratt_amd_truncate_rel_sketch(x, eps, mr, pred);
         int c2[2];
@@ -164,9 +164,9 @@ static void round_rel_cap_role(paratt *x, double eps, int cap, int role) {
         float *dO = dX + LN_FEAT * ns;
         ptt_hip_h2d(dX, feat, sizeof(float) * LN_FEAT * (size_t)ns);
         ptt_hip_mlp_forward(ns, dX, g_learn.dP, dO);
-        float *out = (float *)malloc(sizeof(float) * (size_t)ns);
+        float *out = (float *)ptt_host_malloc(sizeof(float) * (size_t)ns);
         ptt_hip_d2h(out, dO, sizeof(float) * (size_t)ns);
-        int *pred = (int *)calloc((size_t)d + 1, sizeof(int));
+        int *pred = (int *)ptt_host_calloc((size_t)d + 1, sizeof(int));
         for (int i = 1; i < d; i++) {
             long p = lrint(64.0 * out[i - 1]);
             pred[i] = p < 1 ? 1 : (p > 1024 ? 1024 : (int)p);
@@ -205,7 +205,7 @@ static void round_rel_cap(paratt *x, double eps, int cap) { round_rel_cap_role(x
 
 void paratt_amd_mode_sum(paratt *u, const double *sten3, paratt *s) {
     int d = u->numDim;
-    int *rs = (int *)malloc(sizeof(int) * (d + 1));
+    int *rs = (int *)ptt_host_malloc(sizeof(int) * (d + 1));
     for (int k = 0; k <= d; k++) rs[k] = (k == 0 || k == d) ? 1 : 2 * u->x->ttRank[k];
     rank_change_tt(s->x, rs);
     double *dst = ptt_host_coef(sten3, 3);
@@ -255,7 +255,7 @@ void paratt_amd_rxn_burgers_field(paratt *u, paratt *out, int nargs, ...) {
     double h = g_rb.h;
     double d1[3] = {-0.5 / h, 0.0, 0.5 / h};
     double d2[3] = {1.0 / (h * h), -2.0 / (h * h), 1.0 / (h * h)};
-    int *one = (int *)malloc(sizeof(int) * (d + 1));
+    int *one = (int *)ptt_host_malloc(sizeof(int) * (d + 1));
     for (int k = 0; k <= d; k++) one[k] = 1;
     paratt *ones = alloc_ptt(d, u->dim, one);
     for (int k = 0; k < d; k++) ptt_hip_fill(ones->x->coreList[k], u->dim[k], 1.0);
@@ -266,7 +266,7 @@ void paratt_amd_rxn_burgers_field(paratt *u, paratt *out, int nargs, ...) {
     paratt *v0 = ptt_axpby_new(1.0, ones, -1.0, u);
     paratt *v = ptt_axpby_new(1.0, v0, -1.0, s1);
     round_rel_cap_role(v, g_rb.eps, g_rb.cap, ROLE_V);
-    int *rh = (int *)malloc(sizeof(int) * (d + 1));
+    int *rh = (int *)ptt_host_malloc(sizeof(int) * (d + 1));
     for (int k = 0; k <= d; k++) rh[k] = u->x->ttRank[k] * v->x->ttRank[k];
     paratt *hd = alloc_ptt(d, u->dim, rh);
     pattacHadamard(d, u->dim, u->x->ttRank, u->x->coreList, v->x->ttRank, v->x->coreList,
@@ -286,7 +286,7 @@ void paratt_amd_rk4_st(paratt *f_old, paratt *f_new,
                        void (*vecField)(paratt *, paratt *, int, ...), double dt, double eps,
                        int cap) {
     int d = f_old->numDim;
-    int *one = (int *)malloc(sizeof(int) * (d + 1));
+    int *one = (int *)ptt_host_malloc(sizeof(int) * (d + 1));
     for (int k = 0; k <= d; k++) one[k] = 1;
     paratt *k1 = alloc_ptt(d, f_old->dim, one), *k2 = alloc_ptt(d, f_old->dim, one);
     paratt *k3 = alloc_ptt(d, f_old->dim, one), *k4 = alloc_ptt(d, f_old->dim, one);

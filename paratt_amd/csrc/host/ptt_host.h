@@ -19,6 +19,12 @@ void ptt_host_reserve_core(attac_tt *x, int k, long count);
 /* Upload a small host coefficient vector to the COEF scratch slot. */
 double *ptt_host_coef(const double *host, long count);
 void ptt_host_fatal(const char *fn, const char *msg);
+/* Host allocations that print and exit on failure, like the rest of the
+ * library's fatal paths. */
+#include <stddef.h>
+void *ptt_host_malloc(size_t bytes);
+void *ptt_host_calloc(size_t count, size_t size);
+void *ptt_host_realloc(void *p, size_t bytes);
 /* paratt_truncate on a bare attac_tt. */
 void ptt_host_round(attac_tt *x, double tol, const int *maxRank);
 

@@ -339,7 +339,7 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
                        maxr[i], ws->iinfo + 1);
     /* sigma, factor info and SVD info are contiguous: one transfer */
     size_t nb = sizeof(double) * ((size_t)rows + 8) + sizeof(int) * 8;
-    double *hb = (double *)malloc(nb);
+    double *hb = (double *)ptt_host_malloc(nb);
     ptt_hip_d2h(hb, ws->sig, nb);
     double *sig = hb;
     const int *io = (const int *)(hb + rows + 8) + 1;
@@ -449,7 +449,7 @@ static int rtl_step_sketch(attac_tt *x, int i, const int *maxr, double delta,
     ptt_hip_dgemm('N', 'N', rows, (int)w, l, -1.0, QL, rows, B, l, 1.0, E, rows);
     ptt_hip_sumsq(rows, (int)w, E, rows, sw.dinfo + 4);
     ptt_hip_svd_jacobi(l, l, sw.L, l, sw.sig, sw.U, l, sw.V, l, 0.0, delta, l, sw.iinfo + 1);
-    double *hb = (double *)malloc(sizeof(double) * ((size_t)l + 8));
+    double *hb = (double *)ptt_host_malloc(sizeof(double) * ((size_t)l + 8));
     ptt_hip_d2h(hb, sw.sig, sizeof(double) * ((size_t)l + 8));
     const double e2 = hb[l + 4];
     int rho = -1;
@@ -581,12 +581,12 @@ static double round_impl(attac_tt *x, const int *maxRank, double tol, int tol_is
     memset(g_spec, 0, sizeof g_spec);
     g_sketch_counts[0] = g_sketch_counts[1] = 0;
     g_stats.d = d;
-    int *mr = (int *)malloc(sizeof(int) * (d + 1));
+    int *mr = (int *)ptt_host_malloc(sizeof(int) * (d + 1));
     memcpy(mr, maxRank, sizeof(int) * (d + 1)); /* maxRank may alias ttRank */
-    double *inv = (double *)malloc(sizeof(double) * d);
-    double *sq = (double *)malloc(sizeof(double) * d);
-    double *qs = (double *)malloc(sizeof(double) * d);
-    double *rs = (double *)malloc(sizeof(double) * d);
+    double *inv = (double *)ptt_host_malloc(sizeof(double) * d);
+    double *sq = (double *)ptt_host_malloc(sizeof(double) * d);
+    double *qs = (double *)ptt_host_malloc(sizeof(double) * d);
+    double *rs = (double *)ptt_host_malloc(sizeof(double) * d);
     fill_scales(x, inv, sq);
     for (int k = 0; k < d; k++) {
         qs[k] = 1.0;
@@ -656,8 +656,8 @@ void pattacLOrthogonalization(MPI_Comm comm, int d, int *distrib_dim_mat, int *r
     (void)comm; (void)qrTree; (void)int_wrk;
     attac_tt view;
     ptt_host_view(&view, d, distrib_dim_mat, rx, x);
-    double *qs = (double *)malloc(sizeof(double) * d);
-    double *rs = (double *)malloc(sizeof(double) * d);
+    double *qs = (double *)ptt_host_malloc(sizeof(double) * d);
+    double *rs = (double *)ptt_host_malloc(sizeof(double) * d);
     for (int k = 0; k < d; k++) { qs[k] = 1.0; rs[k] = 1.0; }
     if (d >= 2) ltr_sweep(&view, qs, rs);
     ptt_hip_sync();
@@ -671,9 +671,9 @@ void pattacLTruncate(MPI_Comm comm, const int d, int *distrib_dim_mat, int *rx, 
     (void)comm; (void)qrTree; (void)int_wrk; (void)flt_wrk_ptr; (void)flt_sz_ptr;
     attac_tt view;
     ptt_host_view(&view, d, distrib_dim_mat, rx, x);
-    double *ones = (double *)malloc(sizeof(double) * d);
+    double *ones = (double *)ptt_host_malloc(sizeof(double) * d);
     for (int k = 0; k < d; k++) ones[k] = 1.0;
-    int *mr = (int *)malloc(sizeof(int) * (d + 1));
+    int *mr = (int *)ptt_host_malloc(sizeof(int) * (d + 1));
     memcpy(mr, maxr, sizeof(int) * (d + 1));
     if (d >= 2) rtl_sweep(&view, mr, abs_tol, ones);
     ptt_hip_sync();
@@ -684,10 +684,10 @@ void pattacLTruncate(MPI_Comm comm, const int d, int *distrib_dim_mat, int *rx, 
 double paratt_norm(paratt *x_ptt) {
     attac_tt *x = x_ptt->x;
     int d = x->numDim;
-    double *inv = (double *)malloc(sizeof(double) * d);
-    double *sq = (double *)malloc(sizeof(double) * d);
-    double *qs = (double *)malloc(sizeof(double) * d);
-    double *rs = (double *)malloc(sizeof(double) * d);
+    double *inv = (double *)ptt_host_malloc(sizeof(double) * d);
+    double *sq = (double *)ptt_host_malloc(sizeof(double) * d);
+    double *qs = (double *)ptt_host_malloc(sizeof(double) * d);
+    double *rs = (double *)ptt_host_malloc(sizeof(double) * d);
     fill_scales(x, inv, sq);
     for (int k = 0; k < d; k++) {
         qs[k] = sq[k];                       /* leaves core k = sqrt(n_k) * Q_k */

@@ -23,6 +23,20 @@ static void *xmalloc(size_t b) {
     return p;
 }
 
+void *ptt_host_malloc(size_t bytes) { return xmalloc(bytes); }
+
+void *ptt_host_calloc(size_t count, size_t size) {
+    void *p = calloc(count ? count : 1, size ? size : 1);
+    if (!p) ptt_host_fatal("calloc", "host allocation failed");
+    return p;
+}
+
+void *ptt_host_realloc(void *q, size_t bytes) {
+    void *p = realloc(q, bytes ? bytes : 1);
+    if (!p) ptt_host_fatal("realloc", "host allocation failed");
+    return p;
+}
+
 static long core_count(const int *dim, const int *r, int k) {
     return (long)r[k] * dim[k] * r[k + 1];
 }
