@@ -279,8 +279,28 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     for (int i = 0; i < c; i++) nrm0 += (1.0 + 0.01 * (i % 7)) * (1.0 + 0.01 * (i % 7));
     nrm0 = 1.0 / sqrt(nrm0);
     for (int i = tid; i < c; i += nth) pv[i] = (1.0 + 0.01 * (i % 7)) * nrm0;
-    __syncthreads();
-    const int PIT = want_kappa ? 6 : 0;
+    // Gershgorin certificate: the scaled Gram is I + E; if ||E||_inf = e <
+    // 1/2 then kappa_2(Gram) <= (1+e)/(1-e) and the power iterations are
+    // skipped (the bound is reported instead of the estimate).
+    double eoff = 0.0, dummy0 = 0.0;
+    if (want_kappa) {
+        for (int i = tid; i < c; i += nth) {
+            double rs = 0.0;
+            for (int l = 0; l < c; l++)
+                if (l != i) rs += fabs(S[(long)l * c + i]);
+            eoff = fmax(eoff, rs);
+        }
+        for (int o = 32; o > 0; o >>= 1) eoff = fmax(eoff, __shfl_xor(eoff, o));
+        __syncthreads();
+        if ((tid & 63) == 0) s_red[tid >> 6] = eoff;
+        __syncthreads();
+        eoff = 0.0;
+        for (int i = 0; i < nth / 64; i++) eoff = fmax(eoff, s_red[i]);
+        __syncthreads();
+    }
+    (void)dummy0;
+    const bool gersh = want_kappa && eoff < 0.5;
+    const int PIT = (want_kappa && !gersh) ? 6 : 0;
     const int row8 = tid >> 3, sub8 = tid & 7;
     double lmax = 0.0;
     for (int it = 0; it < PIT; it++) {
@@ -445,7 +465,8 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     }
     if (tid == 0) {
         dinfo[0] = sqrt(fr) * sqrt(ft);
-        dinfo[1] = want_kappa ? sqrt(lmax * lminv) : 0.0;
+        // 2-norm condition of Rs = sqrt(kappa_2 of the scaled Gram)
+        dinfo[1] = !want_kappa ? 0.0 : gersh ? sqrt((1.0 + eoff) / (1.0 - eoff)) : sqrt(lmax * lminv);
         dinfo[2] = (double)s_def;
         info[0] = s_def;
     }
