@@ -23,8 +23,19 @@
 /* No MPI in this build: a communicator is a plain integer handle (the same
  * size and passing convention as MPICH's MPI_Comm). */
 typedef int MPI_Comm;
+typedef int MPI_Datatype;
+typedef int MPI_Request;
 #ifndef MPI_COMM_WORLD
 #define MPI_COMM_WORLD ((MPI_Comm)0x44000000)
+#endif
+#ifndef MPI_REQUEST_NULL
+#define MPI_REQUEST_NULL ((MPI_Request)0x2c000000)
+#endif
+#ifndef MPI_DOUBLE
+#define MPI_DOUBLE ((MPI_Datatype)0x4c00080b)
+#endif
+#ifndef MPI_INT
+#define MPI_INT ((MPI_Datatype)0x4c000405)
 #endif
 #endif
 
@@ -87,6 +98,13 @@ double pattacInnerProduct(const MPI_Comm comm, const int *isRedundant,
                           const int d, const int *n, const int *rx,
                           double *const *x, const int *ry, double *const *y,
                           double *work);
+/* Entry x[index[0], ..., index[d-1]] by the chain of matrix-vector products
+ * from the last core to the first.  Replaces pattac.c:82-121; procHolder and
+ * work are ignored (one rank; device workspace is internal).  An index out
+ * of range is fatal. */
+double pattacValue(const MPI_Comm comm, const int *index, const int *procHolder,
+                   const int d, const int *n, const int *rx, double *const *x,
+                   double *work);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,8 @@ void ptt_hip_h2d(void *dst, const void *src, size_t bytes);       /* blocking */
 void ptt_hip_d2h(void *dst, const void *src, size_t bytes);       /* blocking */
 void ptt_hip_d2d(void *dst, const void *src, size_t bytes);       /* async */
 void ptt_hip_zero(void *p, size_t bytes);                         /* async fill kernel */
+/* Blocking copy between any two host or device addresses. */
+void ptt_hip_copy(void *dst, const void *src, size_t bytes);
 void ptt_hip_sync(void);
 /* Grow-only device scratch slots owned by the library.  The returned pointer
  * is valid until the next call for the same slot with a larger size. */
@@ -149,6 +151,24 @@ void ptt_hip_mlp_adam(float *P, const float *grad, float *m, float *v, float lr,
                       int t);
 /* Small c x c triangular product C = A*B (upper times upper), ld c. */
 void ptt_hip_trmm_small(int c, const double *A, const double *B, double *C);
+
+/* ---- quadrature along one mode, entry evaluation (FP64, bit-exact) ----
+ * In place on the r0 x n x r1 core f, sequentially along every fiber
+ * (a, :, b); w and scale are n-vectors on the device.
+ * kind 0: weighted cumulative sum, g[j] = sum_{j'<=j} w[j'] f[j'];
+ * kind 1: trapezoid rule, g[0] = 0, g[j] = g[j-1] + 0.5 w[j] (f[j-1] + f[j]);
+ * kind 2: kind 1 plus f[j] * scale[j] (scale unused otherwise). */
+void ptt_hip_mode_scan(int kind, int r0, int n, int r1, const double *w, const double *scale,
+                       double *f);
+/* C[i + ldc j + sC q] = sum_k A[i + lda k + sA q] * B[k + ldb j + sB q],
+ * summed over k in order from 0.0 (i < M, j < N, q < batch). */
+void ptt_hip_seq_gemm(int M, int N, int K, int batch, const double *A, long lda, long sA,
+                      const double *B, long ldb, long sB, double *C, long ldc, long sC);
+/* Entry (idx[0..d)) of the TT with device cores `cores`: chain of
+ * matrix-vector products from the last core to the first.  vec: device
+ * scratch of max rank doubles.  *out (host) receives the value (syncs). */
+void ptt_hip_tt_entry(int d, const int *n, const int *r, double *const *cores, const int *idx,
+                      double *vec, double *out);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,10 @@ PUBLIC_SYMBOLS = [
     "pattacTruncate", "pattacInnerProduct",
     "paratt_amd_core_upload", "paratt_amd_core_download", "paratt_amd_fill_gaussian",
     "paratt_amd_truncate_rel", "paratt_amd_last_round_stats", "paratt_amd_sync",
+    "attac_tt_truncate", "attac_tt_norm", "send_recv_above", "send_recv_below",
+    "paratt_entry", "pattacValue", "pattac_apply_marginal", "pattac_marginal_cusum",
+    "pattac_marginal_trap", "pattac_scale_weight_sum", "pattac_write_tt_folder",
+    "pattac_load_tt_folder", "read_core_size", "load_core_slice",
 ]
 
 
@@ -107,6 +111,21 @@ def lib():
         L.ptt_hip_prof_get.argtypes = [C.c_char_p, IP, DP]
         L.ptt_hip_prof_names.argtypes = [C.c_char_p, C.c_int]
         L.ptt_hip_timer_stop.restype = C.c_double
+        L.attac_tt_truncate.argtypes = [T, C.c_double, IP]
+        L.attac_tt_norm.restype = C.c_double
+        L.attac_tt_norm.argtypes = [T]
+        L.paratt_entry.restype = C.c_double
+        L.paratt_entry.argtypes = [P, IP]
+        L.pattac_apply_marginal.argtypes = [P, C.POINTER(P), C.c_int, DP]
+        L.pattac_marginal_cusum.argtypes = [P, C.c_int, DP]
+        L.pattac_marginal_trap.argtypes = [P, C.c_int, DP]
+        L.pattac_scale_weight_sum.argtypes = [P, C.c_int, DP, DP]
+        L.pattac_write_tt_folder.argtypes = [C.c_char_p, P]
+        L.pattac_load_tt_folder.restype = P
+        L.pattac_load_tt_folder.argtypes = [C.c_char_p, C.c_int]
+        L.read_core_size.argtypes = [C.c_char_p, IP, IP, IP]
+        L.load_core_slice.argtypes = [C.c_char_p, C.POINTER(C.c_void_p), IP, IP, IP, IP,
+                                      C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -178,6 +197,15 @@ class DevTT:
             pass
 
     @staticmethod
+    def wrap(p):
+        """Take ownership of a `paratt*` returned by the library."""
+        if not p:
+            raise ValueError("NULL paratt")
+        t = DevTT.__new__(DevTT)
+        t.p = p
+        return t
+
+    @staticmethod
     def from_host(host):
         return DevTT(host.n, host.r).upload(host)
 
@@ -243,3 +271,59 @@ def prof_names():
     buf = C.create_string_buffer(8192)
     lib().ptt_hip_prof_names(buf, 8192)
     return [s for s in buf.value.decode().split("\n") if s]
+
+
+# ---------------------------------------------------------------- quadrature, entries, folders
+def entry(x, idx):
+    """x[idx] (paratt_entry)."""
+    return lib().paratt_entry(x.p, _ia(idx))
+
+
+def marginal_trap(x, mode, w):
+    a, ap = _dp(w)
+    lib().pattac_marginal_trap(x.p, mode, ap)
+
+
+def marginal_cusum(x, mode, w):
+    a, ap = _dp(w)
+    lib().pattac_marginal_cusum(x.p, mode, ap)
+
+
+def scale_weight_sum(x, mode, scale, sum_w):
+    a, ap = _dp(scale)
+    b, bp = _dp(sum_w)
+    lib().pattac_scale_weight_sum(x.p, mode, ap, bp)
+
+
+def apply_marginal(x, mode, w):
+    """New DevTT: x with `mode` contracted against w (pattac_apply_marginal)."""
+    a, ap = _dp(w)
+    out = C.POINTER(Paratt)()
+    lib().pattac_apply_marginal(x.p, C.byref(out), mode, ap)
+    return DevTT.wrap(out)
+
+
+def write_folder(x, folder):
+    lib().pattac_write_tt_folder(os.fsencode(folder), x.p)
+
+
+def load_folder(folder, d):
+    p = lib().pattac_load_tt_folder(os.fsencode(folder), d)
+    return DevTT.wrap(p) if p else None
+
+
+def read_core_size(fname):
+    rl, nn, rr = C.c_int(-1), C.c_int(-1), C.c_int(-1)
+    lib().read_core_size(os.fsencode(fname), C.byref(rl), C.byref(nn), C.byref(rr))
+    return rl.value, nn.value, rr.value
+
+
+def truncate_plain(x, tol, maxr=None):
+    """attac_tt_truncate: rounding in the plain Frobenius norm."""
+    mr = _ia(maxr if maxr is not None else x.r)
+    lib().attac_tt_truncate(x.tt, tol, mr)
+
+
+def norm_plain(x):
+    """attac_tt_norm: ||x||_F (leaves x left-orthogonalised)."""
+    return lib().attac_tt_norm(x.tt)

@@ -138,6 +138,13 @@ void ptt_hip_d2d(void *dst, const void *src, size_t bytes) {
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
 }
 
+void ptt_hip_copy(void *dst, const void *src, size_t bytes) {
+    ptt_hip_init();
+    if (!bytes || dst == src) return;
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, g_stream));
+    HIPCHECK(hipStreamSynchronize(g_stream));
+}
+
 void ptt_hip_sync(void) {
     ptt_hip_init();
     HIPCHECK(hipStreamSynchronize(g_stream));

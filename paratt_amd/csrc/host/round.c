@@ -575,7 +575,8 @@ static void fill_scales(const attac_tt *x, double *inv, double *sq) {
 
 /* Shared driver: returns ||x||_n from the sweep (last core's Frobenius norm
  * in the scaled space). */
-static double round_impl(attac_tt *x, const int *maxRank, double tol, int tol_is_rel) {
+static double round_impl_g(attac_tt *x, const int *maxRank, double tol, int tol_is_rel,
+                           int grid) {
     int d = x->numDim;
     memset(&g_stats, 0, sizeof g_stats);
     memset(g_spec, 0, sizeof g_spec);
@@ -588,6 +589,8 @@ static double round_impl(attac_tt *x, const int *maxRank, double tol, int tol_is
     double *qs = (double *)ptt_host_malloc(sizeof(double) * d);
     double *rs = (double *)ptt_host_malloc(sizeof(double) * d);
     fill_scales(x, inv, sq);
+    if (!grid)
+        for (int k = 0; k < d; k++) inv[k] = sq[k] = 1.0;
     for (int k = 0; k < d; k++) {
         qs[k] = 1.0;
         rs[k] = (k + 1 < d) ? inv[k + 1] : 1.0;
@@ -617,6 +620,10 @@ static double round_impl(attac_tt *x, const int *maxRank, double tol, int tol_is
     ptt_hip_sync();
     free(mr); free(inv); free(sq); free(qs); free(rs);
     return nrm;
+}
+
+static double round_impl(attac_tt *x, const int *maxRank, double tol, int tol_is_rel) {
+    return round_impl_g(x, maxRank, tol, tol_is_rel, 1);
 }
 
 /* ---- public entry points ---- */
@@ -716,3 +723,26 @@ double paratt_norm(paratt *x_ptt) {
 
 /* Rounding of a bare attac_tt (used by attac_exp_euler_st). */
 void ptt_host_round(attac_tt *x, double tol, const int *maxRank) { round_impl(x, maxRank, tol, 0); }
+
+void attac_tt_truncate(attac_tt *x_tt, double tol, int *maxRank) {
+    round_impl_g(x_tt, maxRank, tol, 0, 0);
+}
+
+double attac_tt_norm(attac_tt *x) {
+    int d = x->numDim;
+    double *dss = (double *)ptt_hip_scratch(PTT_WS_INT, sizeof(int) * 64) + 24;
+    double ss = 0.0;
+    if (d >= 2) {
+        double *ones = (double *)ptt_host_malloc(sizeof(double) * d);
+        for (int k = 0; k < d; k++) ones[k] = 1.0;
+        ltr_sweep(x, ones, ones);
+        free(ones);
+        int rl = x->ttRank[d - 1];
+        ptt_hip_sumsq(rl, x->dim[d - 1], x->coreList[d - 1], rl, dss);
+    } else {
+        long cnt = (long)x->ttRank[0] * x->dim[0] * x->ttRank[1];
+        ptt_hip_sumsq(1, (int)cnt, x->coreList[0], 1, dss);
+    }
+    ptt_hip_d2h(&ss, dss, sizeof(double));
+    return sqrt(ss);
+}

@@ -14,6 +14,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
+#include <sys/stat.h>
 #include <mpi.h>
 #include "paratt.h"
 #include "paratt_ode.h"
@@ -331,6 +333,138 @@ static void euler_case(void) {
     free_ptt(f); free_ptt(fn); free_ptt(wk);
 }
 
+/* ---------------- quadrature, marginals, entries, TT folders ---------------- */
+static void put_text_file(const char *path) {
+    FILE *f = fopen(path, "r");
+    if (!f) { fprintf(stderr, "cannot read %s\n", path); exit(1); }
+    printf("\"");
+    int ch;
+    while ((ch = fgetc(f)) != EOF) {
+        if (ch == '\n') printf("\\n");
+        else if (ch == '"' || ch == '\\') printf("\\%c", ch);
+        else putchar(ch);
+    }
+    printf("\"");
+    fclose(f);
+}
+
+static void quad_cases(void) {
+    int d = 4, n[4] = {5, 6, 4, 3}, r[5] = {1, 3, 2, 4, 1};
+    paratt *x = rand_ptt(d, n, r, 61, 1.0);
+    key("quad_x"); put_tt(x);
+    char name[64];
+    for (int mode = 0; mode < d; mode++) {
+        double w[8], s[8];
+        gauss_fill(62, mode, 1.0, n[mode], w);
+        gauss_fill(63, mode, 1.0, n[mode], s);
+        snprintf(name, sizeof name, "quad_w_%d", mode); key(name); put_dbls(w, n[mode]);
+        snprintf(name, sizeof name, "quad_s_%d", mode); key(name); put_dbls(s, n[mode]);
+        long cnt = (long)r[mode] * n[mode] * r[mode + 1];
+        paratt *c = clone_ptt(x);
+        pattac_marginal_trap(c, mode, w);
+        snprintf(name, sizeof name, "quad_trap_%d", mode); key(name);
+        put_dbls(c->x->coreList[mode], cnt);
+        free_ptt(c);
+        c = clone_ptt(x);
+        pattac_marginal_cusum(c, mode, w);
+        snprintf(name, sizeof name, "quad_cusum_%d", mode); key(name);
+        put_dbls(c->x->coreList[mode], cnt);
+        free_ptt(c);
+        c = clone_ptt(x);
+        pattac_scale_weight_sum(c, mode, s, w);
+        snprintf(name, sizeof name, "quad_sws_%d", mode); key(name);
+        put_dbls(c->x->coreList[mode], cnt);
+        free_ptt(c);
+        c = clone_ptt(x);
+        paratt *m = NULL;
+        pattac_apply_marginal(c, &m, mode, w);
+        snprintf(name, sizeof name, "quad_marginal_%d", mode); key(name);
+        put_tt(m);
+        free_ptt(m);
+        free_ptt(c);
+    }
+    /* entries through the library's own paratt_entry */
+    {
+        int idx[4];
+        key("quad_entry");
+        printf("{\"idx\": [");
+        for (int t = 0; t < 12; t++) {
+            for (int k = 0; k < d; k++) idx[k] = (int)(mix64(7000ULL + 31ULL * t + k) % (uint64_t)n[k]);
+            if (t) printf(",");
+            put_ints(idx, d);
+        }
+        printf("], \"val\": [");
+        for (int t = 0; t < 12; t++) {
+            for (int k = 0; k < d; k++) idx[k] = (int)(mix64(7000ULL + 31ULL * t + k) % (uint64_t)n[k]);
+            printf("%s%.17g", t ? "," : "", paratt_entry(x, idx));
+        }
+        printf("]}");
+    }
+    /* TT folders.  The reference's writer deadlocks at one MPI rank as soon
+     * as a core has right rank > 1 (blocking MPI_Send to itself,
+     * paratt.c:1684-1686), so it writes a rank-1 TT here; the rank > 1
+     * case is written in the documented format by write_plain() below and
+     * read back with the reference's loader, which must reproduce x. */
+    {
+        char dir[] = "/tmp/ptt_golden_ioXXXXXX";
+        if (!mkdtemp(dir)) { perror("mkdtemp"); exit(1); }
+        char folder[128], path[192];
+        int n1[3] = {5, 4, 3}, r1[4] = {1, 1, 1, 1};
+        paratt *y = rand_ptt(3, n1, r1, 64, 1.0);
+        key("io_y_rank1"); put_tt(y);
+        snprintf(folder, sizeof folder, "%s/rank1", dir);
+        pattac_write_tt_folder(folder, y);
+        key("io_files_rank1");
+        printf("[");
+        for (int k = 0; k < 3; k++) {
+            snprintf(path, sizeof path, "%s/core_%d.cr", folder, k + 1);
+            if (k) printf(",");
+            put_text_file(path);
+            remove(path);
+        }
+        printf("]");
+        rmdir(folder);
+        free_ptt(y);
+        snprintf(folder, sizeof folder, "%s/plain", dir);
+        mkdir(folder, 0777);
+        for (int k = 0; k < d; k++) {
+            snprintf(path, sizeof path, "%s/core_%d.cr", folder, k + 1);
+            FILE *f = fopen(path, "w");
+            fprintf(f, "%d,%d,%d\n", r[k], n[k], r[k + 1]);
+            for (long e = 0; e < (long)r[k] * n[k] * r[k + 1]; e++)
+                fprintf(f, "%+1.16e\n", x->x->coreList[k][e]);
+            fclose(f);
+        }
+        paratt *z = pattac_load_tt_folder(folder, d);
+        int same = 1;
+        for (int k = 0; k <= d; k++) same &= (z->x->ttRank[k] == r[k]);
+        for (int k = 0; k < d && same; k++)
+            same &= memcmp(z->x->coreList[k], x->x->coreList[k],
+                           sizeof(double) * (size_t)r[k] * n[k] * r[k + 1]) == 0;
+        key("io_reload_identical"); printf("%d", same);
+        /* a slice [1, 4) of mode 1 through the reference's load_core_slice */
+        {
+            double *buf = NULL;
+            int cap = 0, rl = 0, nn = 0, rr = 0;
+            snprintf(path, sizeof path, "%s/core_2.cr", folder);
+            load_core_slice(path, &buf, &cap, &rl, &nn, &rr, 1, 4);
+            key("io_slice_core2_1_4");
+            printf("{\"r0\": %d, \"n\": %d, \"r1\": %d, \"v\": ", rl, nn, rr);
+            put_dbls(buf, (long)rl * nn * rr);
+            printf("}");
+            free(buf);
+        }
+        free_ptt(z);
+        for (int k = 0; k < d; k++) {
+            snprintf(path, sizeof path, "%s/core_%d.cr", folder, k + 1);
+            remove(path);
+        }
+        rmdir(folder);
+        rmdir(dir);
+    }
+    free_ptt(x);
+}
+
 int main(int argc, char **argv) {
     MPI_Init(&argc, &argv);
     printf("{");
@@ -366,6 +500,7 @@ int main(int argc, char **argv) {
     }
     ops_cases();
     euler_case();
+    quad_cases();
     printf("\n}\n");
     MPI_Finalize();
     return 0;

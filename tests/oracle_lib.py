@@ -324,3 +324,64 @@ def gaussian_ic(d, n, centre=np.pi, width=0.5):
     for k in range(d):
         cores.append(np.exp(-(xs - cen[k]) ** 2 / width))
     return HostTT([n] * d, [1] * (d + 1), cores)
+
+
+# ---------------------------------------------------------------- quadrature and entries
+def mode_scan(x, mode, kind, w, scale=None):
+    """Oracle of pattac_marginal_cusum (kind 0), pattac_marginal_trap (kind 1)
+    and pattac_scale_weight_sum (kind 2) on a copy of x."""
+    a, ap = _dp(w)
+    s, sp = _dp(scale if scale is not None else np.zeros(len(w)))
+    y = x.copy()
+    c = y.cores[mode]
+    lib().o_mode_scan(C.c_int(kind), C.c_int(y.r[mode]), C.c_int(y.n[mode]),
+                      C.c_int(y.r[mode + 1]), ap, sp, c.ctypes.data_as(C.POINTER(C.c_double)))
+    return y
+
+
+def apply_marginal(x, mode, w):
+    """Oracle of pattac_apply_marginal: a HostTT with d-1 modes."""
+    a, ap = _dp(w)
+    d = x.d
+    n_out = [x.n[k] for k in range(d) if k != mode]
+    if mode < d - 1:
+        r_out = x.r[: mode + 1] + x.r[mode + 2:]
+    else:
+        r_out = x.r[: d - 1] + [1]
+    outs = [np.zeros(max(r_out[k] * n_out[k] * r_out[k + 1], 1)) for k in range(d - 1)]
+    rr = np.zeros(d, dtype=np.int32)
+    xs = [np.ascontiguousarray(c) for c in x.cores]
+    lib().o_apply_marginal(C.c_int(d), _ip(x.n).ctypes.data_as(C.POINTER(C.c_int)),
+                           _ip(x.r).ctypes.data_as(C.POINTER(C.c_int)), _ptrs(xs),
+                           C.c_int(mode), ap, _ptrs(outs), rr.ctypes.data_as(C.POINTER(C.c_int)))
+    assert list(rr) == r_out
+    return HostTT(n_out, r_out, [outs[k][: r_out[k] * n_out[k] * r_out[k + 1]]
+                                 for k in range(d - 1)])
+
+
+def entry(x, idx):
+    """Oracle of paratt_entry (right-to-left chain)."""
+    xs = [np.ascontiguousarray(c) for c in x.cores]
+    f = lib().o_entry
+    f.restype = C.c_double
+    return f(C.c_int(x.d), _ip(x.n).ctypes.data_as(C.POINTER(C.c_int)),
+             _ip(x.r).ctypes.data_as(C.POINTER(C.c_int)), _ptrs(xs),
+             _ip(idx).ctypes.data_as(C.POINTER(C.c_int)))
+
+
+def truncate_plain(x, tol, maxr=None):
+    """attac_tt_truncate on a copy of x: LTR orthogonalisation then RTL
+    truncation with absolute tolerance tol, no grid scaling
+    (attac.c:909-925 intent; pattac.c:244-522 steps)."""
+    y = x.copy()
+    d = y.d
+    r = _ip(y.r)
+    mr = _ip(maxr if maxr is not None else y.r)
+    lib().o_lorth(C.c_int(d), _ip(y.n).ctypes.data_as(C.POINTER(C.c_int)),
+                  r.ctypes.data_as(C.POINTER(C.c_int)), _ptrs(y.cores))
+    lib().o_ltruncate(C.c_int(d), _ip(y.n).ctypes.data_as(C.POINTER(C.c_int)),
+                      r.ctypes.data_as(C.POINTER(C.c_int)), _ptrs(y.cores),
+                      mr.ctypes.data_as(C.POINTER(C.c_int)), C.c_double(tol), None, C.c_int(0),
+                      None)
+    y.r = [int(v) for v in r]
+    return y.trimmed()
