@@ -17,6 +17,16 @@
 
 #include <stdlib.h>
 
+/* Like the reference headers (pattac.h:21-24), pull in MPI when the
+ * compiler can see it, so MPI user code keeps compiling against these
+ * headers unchanged; define PARATT_AMD_NO_MPI to use the one-rank handles
+ * below instead.  The library itself is built without MPI. */
+#if !defined(MPI_VERSION) && !defined(PARATT_AMD_NO_MPI) && defined(__has_include)
+#if __has_include(<mpi.h>)
+#include <mpi.h>
+#endif
+#endif
+
 #if defined(MPI_VERSION)
 /* The caller already included an MPI header: use its communicator type. */
 #else

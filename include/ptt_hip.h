@@ -28,6 +28,11 @@ void ptt_hip_d2d(void *dst, const void *src, size_t bytes);       /* async */
 void ptt_hip_zero(void *p, size_t bytes);                         /* async fill kernel */
 /* Blocking copy between any two host or device addresses. */
 void ptt_hip_copy(void *dst, const void *src, size_t bytes);
+/* 1 when the library runs in host-visible mode (environment variable
+ * PARATT_AMD_HOST_VISIBLE=1 at first use): every allocation above is then
+ * pinned host memory that the GPU accesses through the same address, so
+ * host code may read and write cores directly between calls. */
+int ptt_hip_host_visible(void);
 void ptt_hip_sync(void);
 /* Grow-only device scratch slots owned by the library.  The returned pointer
  * is valid until the next call for the same slot with a larger size. */

@@ -180,6 +180,12 @@ void ptt_hip_mlp_forward(int nsamp, const float *X, const float *P, float *out) 
 void ptt_hip_mlp_grad(int nsamp, const float *X, const float *Y, const float *P, float *grad,
                       float *loss) {
     ptt_hip_init();
+    if (ptt_hip_host_visible()) {
+        // the gradient is accumulated with device atomics, which host memory
+        // reached over the host link does not guarantee
+        fprintf(stderr, "ptt_hip_mlp_grad: not available with PARATT_AMD_HOST_VISIBLE\n");
+        exit(1);
+    }
     PTT_LAUNCH("zero_f", k_zero_f, dim3(cdiv(NPARAM + 1, 256)), dim3(256), 0, grad, NPARAM);
     PTT_LAUNCH("zero_f", k_zero_f, dim3(1), dim3(32), 0, loss, 1);
     if (nsamp <= 0) return;

@@ -10,6 +10,14 @@ namespace ptt {
 
 static bool g_init = false;
 hipStream_t g_stream = nullptr;
+// Host-visible mode (environment PARATT_AMD_HOST_VISIBLE=1, read once at
+// init): every library allocation is pinned host memory that the GPU reads
+// and writes through the same address, so host code may dereference
+// coreList[k] between calls as with the reference's host-resident cores.
+// Kernels then move their data over the host link: a compatibility mode for
+// callers that touch cores on the host, not a fast path.
+static bool g_host_visible = false;
+static hipMemcpyKind copy_kind(hipMemcpyKind k) { return g_host_visible ? hipMemcpyDefault : k; }
 
 void fatal_hip(hipError_t e, const char *what, const char *file, int line) {
     fprintf(stderr, "paratt_amd: HIP error '%s' (%d) in %s at %s:%d\n",
@@ -96,6 +104,8 @@ void ptt_hip_init(void) {
     }
     // Default-priority, non-blocking stream owned by the library.
     HIPCHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    const char *hv = getenv("PARATT_AMD_HOST_VISIBLE");
+    g_host_visible = hv != nullptr && hv[0] != '\0' && hv[0] != '0';
     g_init = true;
 }
 
@@ -103,10 +113,11 @@ void *ptt_hip_malloc(size_t bytes) {
     ptt_hip_init();
     void *p = nullptr;
     if (bytes == 0) bytes = 8;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = g_host_visible ? hipHostMalloc(&p, bytes, hipHostMallocDefault)
+                                  : hipMalloc(&p, bytes);
     if (e != hipSuccess) {
-        fprintf(stderr, "paratt_amd: hipMalloc(%zu bytes) failed: %s\n", bytes,
-                hipGetErrorString(e));
+        fprintf(stderr, "paratt_amd: %s(%zu bytes) failed: %s\n",
+                g_host_visible ? "hipHostMalloc" : "hipMalloc", bytes, hipGetErrorString(e));
         exit(1);
     }
     return p;
@@ -115,27 +126,35 @@ void *ptt_hip_malloc(size_t bytes) {
 void ptt_hip_free(void *p) {
     if (!p) return;
     HIPCHECK(hipStreamSynchronize(g_stream));
-    HIPCHECK(hipFree(p));
+    if (g_host_visible)
+        HIPCHECK(hipHostFree(p));
+    else
+        HIPCHECK(hipFree(p));
+}
+
+int ptt_hip_host_visible(void) {
+    ptt_hip_init();
+    return g_host_visible ? 1 : 0;
 }
 
 void ptt_hip_h2d(void *dst, const void *src, size_t bytes) {
     ptt_hip_init();
     if (!bytes) return;
-    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, copy_kind(hipMemcpyHostToDevice), g_stream));
     HIPCHECK(hipStreamSynchronize(g_stream));
 }
 
 void ptt_hip_d2h(void *dst, const void *src, size_t bytes) {
     ptt_hip_init();
     if (!bytes) return;
-    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, copy_kind(hipMemcpyDeviceToHost), g_stream));
     HIPCHECK(hipStreamSynchronize(g_stream));
 }
 
 void ptt_hip_d2d(void *dst, const void *src, size_t bytes) {
     ptt_hip_init();
     if (!bytes || dst == src) return;
-    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, copy_kind(hipMemcpyDeviceToDevice), g_stream));
 }
 
 void ptt_hip_copy(void *dst, const void *src, size_t bytes) {

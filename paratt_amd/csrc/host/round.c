@@ -32,6 +32,8 @@
 #include "ptt_host.h"
 
 #define KAPPA_ONE_PASS 30.0
+/* Largest accepted estimate of the Gram route's subspace error (rtl_step_fast). */
+#define FAST_EST_MAX 1.0e-13
 #define CHOLQR_MAX_PASSES 6
 #define KAPPA_TWO_PASS 1.0e6
 #define SKETCH_OVERSAMPLE 8
@@ -325,6 +327,23 @@ static int rtl_step_robust(attac_tt *x, int i, const int *maxr, double delta,
  * M itself, so the tail rule is decided exactly as with an orthogonal LQ.
  * Returns rho, or -1 when the step must be redone by the robust path (the
  * core is left unchanged in that case). */
+/* Accuracy of the Gram route against an orthogonal (LQ) factorization for a
+ * cut after rho of the rows singular values sig (descending): the Gram's
+ * rounding, ~ rows*u*s0^2, turns the kept subspace by about
+ * est = rows*u*s0^2 / (s_{rho-1}^2 - s_rho^2) (when nothing is cut, est is
+ * the new rows' loss of orthonormality), which exceeds the LQ route's
+ * rows*u*s0 / (s_{rho-1} - s_rho) by the factor s0 / (s_{rho-1} + s_rho).
+ * The Gram route is kept only when est is negligible or that factor is at
+ * most 2, so its result stays as close to the reference's orthogonal
+ * rounding as the problem's own sensitivity allows. */
+static int gram_route_accurate(const double *sig, int rows, int rho) {
+    const double s0 = sig[0], sl = sig[rho - 1], sn = (rho < rows) ? sig[rho] : 0.0;
+    const double gap2 = (sl - sn) * (sl + sn);
+    if (!(s0 > 0.0) || !(gap2 > 0.0)) return 0;
+    const double est = rows * 0x1.0p-53 * s0 * s0 / gap2;
+    return est <= FAST_EST_MAX || s0 <= 2.0 * (sl + sn);
+}
+
 static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
                          const double *nscale, step_ws *ws) {
     int *r = x->ttRank, *n = x->dim;
@@ -349,9 +368,7 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
     int rho0 = cap < numrank ? cap : numrank;
     if (rho0 < 1) rho0 = 1;
     const double s0 = sig[0];
-    /* kept directions must be well conditioned: orthonormality of the new
-     * rows is ~ u * (s0 / s_{rho0-1})^2 */
-    if (!(s0 > 0.0) || sig[rho0 - 1] < 1e-2 * s0) { free(hb); return -1; }
+    if (!(s0 > 0.0)) { free(hb); return -1; }
     int h_rho0 = rho0;
     ptt_hip_h2d(ws->iinfo + 8, &h_rho0, sizeof(int));
     double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)rho0 * w);
@@ -381,6 +398,7 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
             if (rho < 1) rho = 1;
         }
     }
+    if (trust) trust = gram_route_accurate(sig, rows, rho);
     if (!trust) {
         /* undo: M was overwritten by the residual, restore it */
         ptt_hip_dgemm('N', 'N', rows, (int)w, rho0, 1.0 / nscale[i], ws->US, rows, tmp, rho0, 1.0,
