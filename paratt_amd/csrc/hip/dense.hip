@@ -279,6 +279,8 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     for (int i = 0; i < c; i++) nrm0 += (1.0 + 0.01 * (i % 7)) * (1.0 + 0.01 * (i % 7));
     nrm0 = 1.0 / sqrt(nrm0);
     for (int i = tid; i < c; i += nth) pv[i] = (1.0 + 0.01 * (i % 7)) * nrm0;
+    // S and pv are read across threads from here on
+    __syncthreads();
     // Gershgorin certificate: the scaled Gram is I + E; if ||E||_inf = e <
     // 1/2 then kappa_2(Gram) <= (1+e)/(1-e) and the power iterations are
     // skipped (the bound is reported instead of the estimate).

@@ -16,8 +16,32 @@ hipStream_t g_stream = nullptr;
 // coreList[k] between calls as with the reference's host-resident cores.
 // Kernels then move their data over the host link: a compatibility mode for
 // callers that touch cores on the host, not a fast path.
+// In that mode the memory is allocated coherent (fine-grained), so host
+// stores are seen by later kernels and kernel stores by the host after a
+// stream synchronisation; copies are made stream-ordered explicitly: host
+// side ones wait for the stream and then copy on the host, device side ones
+// run as a copy kernel on the library stream.
 static bool g_host_visible = false;
-static hipMemcpyKind copy_kind(hipMemcpyKind k) { return g_host_visible ? hipMemcpyDefault : k; }
+
+__global__ void k_copy_bytes(unsigned char *__restrict__ dst, const unsigned char *__restrict__ src,
+                             size_t bytes) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ((((uintptr_t)dst | (uintptr_t)src) & 7) == 0) {
+        const size_t nw = bytes / 8;
+        uint64_t *d8 = (uint64_t *)dst;
+        const uint64_t *s8 = (const uint64_t *)src;
+        for (size_t i = t0; i < nw; i += stride) d8[i] = s8[i];
+        for (size_t i = nw * 8 + t0; i < bytes; i += stride) dst[i] = src[i];
+    } else {
+        for (size_t i = t0; i < bytes; i += stride) dst[i] = src[i];
+    }
+}
+
+static void host_mode_copy(void *dst, const void *src, size_t bytes) {
+    HIPCHECK(hipStreamSynchronize(g_stream));
+    memcpy(dst, src, bytes);
+}
 
 void fatal_hip(hipError_t e, const char *what, const char *file, int line) {
     fprintf(stderr, "paratt_amd: HIP error '%s' (%d) in %s at %s:%d\n",
@@ -113,7 +137,7 @@ void *ptt_hip_malloc(size_t bytes) {
     ptt_hip_init();
     void *p = nullptr;
     if (bytes == 0) bytes = 8;
-    hipError_t e = g_host_visible ? hipHostMalloc(&p, bytes, hipHostMallocDefault)
+    hipError_t e = g_host_visible ? hipHostMalloc(&p, bytes, hipHostMallocCoherent)
                                   : hipMalloc(&p, bytes);
     if (e != hipSuccess) {
         fprintf(stderr, "paratt_amd: %s(%zu bytes) failed: %s\n",
@@ -140,26 +164,36 @@ int ptt_hip_host_visible(void) {
 void ptt_hip_h2d(void *dst, const void *src, size_t bytes) {
     ptt_hip_init();
     if (!bytes) return;
-    HIPCHECK(hipMemcpyAsync(dst, src, bytes, copy_kind(hipMemcpyHostToDevice), g_stream));
+    if (g_host_visible) return host_mode_copy(dst, src, bytes);
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
     HIPCHECK(hipStreamSynchronize(g_stream));
 }
 
 void ptt_hip_d2h(void *dst, const void *src, size_t bytes) {
     ptt_hip_init();
     if (!bytes) return;
-    HIPCHECK(hipMemcpyAsync(dst, src, bytes, copy_kind(hipMemcpyDeviceToHost), g_stream));
+    if (g_host_visible) return host_mode_copy(dst, src, bytes);
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
     HIPCHECK(hipStreamSynchronize(g_stream));
 }
 
 void ptt_hip_d2d(void *dst, const void *src, size_t bytes) {
     ptt_hip_init();
     if (!bytes || dst == src) return;
-    HIPCHECK(hipMemcpyAsync(dst, src, bytes, copy_kind(hipMemcpyDeviceToDevice), g_stream));
+    if (g_host_visible) {
+        unsigned blocks = (unsigned)((bytes / 8 + 255) / 256);
+        blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
+        PTT_LAUNCH("copy_bytes", k_copy_bytes, dim3(blocks), dim3(256), 0, (unsigned char *)dst,
+                   (const unsigned char *)src, bytes);
+        return;
+    }
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
 }
 
 void ptt_hip_copy(void *dst, const void *src, size_t bytes) {
     ptt_hip_init();
     if (!bytes || dst == src) return;
+    if (g_host_visible) return host_mode_copy(dst, src, bytes);
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, g_stream));
     HIPCHECK(hipStreamSynchronize(g_stream));
 }
