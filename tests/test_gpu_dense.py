@@ -185,3 +185,27 @@ def test_svd_zero_columns_and_rank_deficiency():
     assert np.max(np.abs(s - sref)) < 1e-13 * sref[0] * 4
     assert iv[0] == 28 and s[28] == 0.0 and s[29] == 0.0
     assert np.max(np.abs((U * s) @ V.T - A)) < 1e-12 * 30
+
+
+@pytest.mark.parametrize("c", [5, 16, 40, 100])
+def test_chol_repeatable(c):
+    # every thread of the factor kernel must see the scaled Gram before it
+    # reads it (a missing barrier made the condition estimate, and so the
+    # number of Cholesky-QR passes, vary between identical calls): repeated
+    # calls on one input must agree bit for bit, estimate included
+    L = _lib()
+    rng = np.random.default_rng(c)
+    A = rng.standard_normal((3 * c, c)) * np.logspace(0, -3, c)
+    G = A.T @ A
+    dG = DevBuf(G.ravel(order="F"))
+    dR, dT = DevBuf(np.zeros(c * c)), DevBuf(np.zeros(c * c))
+    di, dd = DevBuf(np.zeros(2)), DevBuf(np.zeros(4))
+    first = None
+    for rep in range(300):
+        L.ptt_hip_chol_qr_factor(c, dG.ptr, 0.0, 1e-14, dR.ptr, dT.ptr, di.ptr, dd.ptr, 1)
+        got = (dR.get().tobytes(), dT.get().tobytes(), dd.get()[:2].tobytes())
+        if first is None:
+            first = got
+        assert got == first, rep
+    for b in (dG, dR, dT, di, dd):
+        b.free()
