@@ -154,6 +154,13 @@ void ptt_hip_mlp_grad(int nsamp, const float *X, const float *Y, const float *P,
                       float *grad, float *loss);
 void ptt_hip_mlp_adam(float *P, const float *grad, float *m, float *v, float lr,
                       int t);
+/* Gp = G[perm, perm] for the c x c matrix G (ld c) with perm ordering the
+ * diagonal descending (ties by index), perm (c ints, device) written too;
+ * c <= 1024. */
+void ptt_hip_sym_sort_perm(int c, const double *G, double *Gp, int *perm);
+/* dst[perm[i] + j*ldd] = src[i + j*lds] for i < m, j < n. */
+void ptt_hip_unpermute_rows(int m, int n, const int *perm, const double *src, long lds,
+                            double *dst, long ldd);
 /* Small c x c triangular product C = A*B (upper times upper), ld c. */
 void ptt_hip_trmm_small(int c, const double *A, const double *B, double *C);
 

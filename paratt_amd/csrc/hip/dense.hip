@@ -884,6 +884,52 @@ __global__ void k_svd_factors(int m, int rhomax, const int *__restrict__ drho,
     }
 }
 
+// Symmetric permutation of a c x c Gram by descending diagonal (ties by
+// index; NaN sorts last): perm[pos] = j, Gp = G[perm, perm].  Cholesky of Gp
+// then yields a factor whose columns are ordered by decreasing norm, which
+// cuts the one-sided Jacobi sweeps on its transpose (the diagonal-pivoting
+// preconditioner of Drmac and Veselic, SIAM J. Matrix Anal. Appl. 29, 2008,
+// applied statically).
+constexpr int SORT_MAXC = 1024;
+__global__ __launch_bounds__(256) void k_sym_sort_perm(int c, const double *__restrict__ G,
+                                                        double *__restrict__ Gp,
+                                                        int *__restrict__ perm) {
+    __shared__ double sd[SORT_MAXC];
+    __shared__ int sp[SORT_MAXC];
+    for (int j = threadIdx.x; j < c; j += blockDim.x) {
+        double v = G[j + (long)j * c];
+        sd[j] = (v == v) ? v : -INFINITY;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < c; j += blockDim.x) {
+        const double dj = sd[j];
+        int pos = 0;
+        for (int i = 0; i < c; i++) {
+            const double di = sd[i];
+            pos += (di > dj) || (di == dj && i < j);
+        }
+        sp[pos] = j;
+        perm[pos] = j;
+    }
+    __syncthreads();
+    for (long e = threadIdx.x; e < (long)c * c; e += blockDim.x) {
+        const int i = (int)(e % c), k = (int)(e / c);
+        Gp[e] = G[sp[i] + (long)sp[k] * c];
+    }
+}
+
+// dst[perm[i] + j*ldd] = src[i + j*lds], i < m, j < n.
+__global__ void k_unpermute_rows(int m, int n, const int *__restrict__ perm,
+                                 const double *__restrict__ src, long lds,
+                                 double *__restrict__ dst, long ldd) {
+    const long total = (long)m * n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const long j = e / m, i = e - j * m;
+        dst[perm[i] + j * ldd] = src[i + j * lds];
+    }
+}
+
 template <typename KF>
 void set_lds_attr(KF kern, int ndoubles) {
     HIPCHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -989,6 +1035,25 @@ void ptt_hip_svd_factors(int m, int rhomax, const int *drho, const double *U, lo
     if (total <= 0) return;
     PTT_LAUNCH("svd_factors", k_svd_factors, dim3(stream_blocks(total, 256)), dim3(256), 0, m,
                rhomax, drho, U, ldu, sig, wscale, W, US);
+}
+
+void ptt_hip_sym_sort_perm(int c, const double *G, double *Gp, int *perm) {
+    ptt_hip_init();
+    if (c <= 0) return;
+    if (c > SORT_MAXC) {
+        fprintf(stderr, "ptt_hip_sym_sort_perm: c = %d exceeds %d\n", c, SORT_MAXC);
+        exit(1);
+    }
+    PTT_LAUNCH("sym_sort_perm", k_sym_sort_perm, dim3(1), dim3(256), 0, c, G, Gp, perm);
+}
+
+void ptt_hip_unpermute_rows(int m, int n, const int *perm, const double *src, long lds,
+                            double *dst, long ldd) {
+    ptt_hip_init();
+    long total = (long)m * n;
+    if (total <= 0) return;
+    PTT_LAUNCH("unpermute_rows", k_unpermute_rows, dim3(stream_blocks(total, 256)), dim3(256), 0,
+               m, n, perm, src, lds, dst, ldd);
 }
 
 void ptt_hip_trmm_small(int c, const double *A, const double *B, double *C) {

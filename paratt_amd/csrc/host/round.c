@@ -351,16 +351,26 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
     long w = (long)n[i] * r[i + 1];
     double *M = x->coreList[i];
     ptt_hip_gram('N', rows, w, M, rows, ws->G);
-    chol_out o = run_chol_k(rows, ws->G, 0.0, ws->R1, ws->T1, ws, 0);
-    if (i < 64) g_stats.kappa_rtl[i] = o.kappa;
+    /* Factor the Gram with its indices ordered by decreasing diagonal: the
+     * columns of L = R^T then come by decreasing norm, which roughly halves
+     * the Jacobi sweeps on graded bond spectra.  L's rows are in that order,
+     * so U's rows are put back afterwards. */
+    int *perm = (int *)ws->Rtmp;
+    ptt_hip_sym_sort_perm(rows, ws->G, ws->T2, perm);
+    ptt_hip_chol_qr_factor(rows, ws->T2, 0.0, tiny_pivot(), ws->R1, ws->T1, ws->iinfo, ws->dinfo,
+                           0);
     ptt_hip_transpose(rows, rows, ws->R1, rows, ws->L, rows, 1.0);
     ptt_hip_svd_jacobi(rows, rows, ws->L, rows, ws->sig, ws->U, rows, NULL, rows, 0.0, delta,
                        maxr[i], ws->iinfo + 1);
+    ptt_hip_unpermute_rows(rows, rows, perm, ws->U, rows, ws->V, rows);
     /* sigma, factor info and SVD info are contiguous: one transfer */
     size_t nb = sizeof(double) * ((size_t)rows + 8) + sizeof(int) * 8;
     double *hb = (double *)ptt_host_malloc(nb);
     ptt_hip_d2h(hb, ws->sig, nb);
     double *sig = hb;
+    /* Frobenius condition of the scaled factor / rows: a lower bound of its
+     * 2-norm condition (statistics only) */
+    if (i < 64) g_stats.kappa_rtl[i] = hb[rows] / rows;
     const int *io = (const int *)(hb + rows + 8) + 1;
     int numrank = io[0];
     if (i < 64) { g_stats.svd_sweeps[i] = io[2]; g_stats.numrank[i] = numrank; }
@@ -372,7 +382,7 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
     int h_rho0 = rho0;
     ptt_hip_h2d(ws->iinfo + 8, &h_rho0, sizeof(int));
     double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)rho0 * w);
-    ptt_hip_svd_factors(rows, rho0, ws->iinfo + 8, ws->U, rows, ws->sig, nscale[i], ws->W, ws->US);
+    ptt_hip_svd_factors(rows, rho0, ws->iinfo + 8, ws->V, rows, ws->sig, nscale[i], ws->W, ws->US);
     ptt_hip_dgemm('N', 'N', rho0, (int)w, rows, 1.0, ws->W, rho0, M, rows, 0.0, tmp, rho0);
     /* residual in place: M <- M - U Sigma (new core)/sqrt(n_i); tail^2 = ||M||^2 */
     ptt_hip_dgemm('N', 'N', rows, (int)w, rho0, -1.0 / nscale[i], ws->US, rows, tmp, rho0, 1.0, M,
@@ -389,13 +399,21 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
          * below the Gram's resolution, which this step cannot resolve. */
         trust = (rho0 == cap);
     } else {
-        /* the cut moves inside the kept list: trustworthy only when delta is
-         * far above the Gram's rounding floor */
+        /* The tail beyond the kept list is below delta, so the cut may move
+         * inside the list.  The Gram's squared singular values carry an
+         * absolute error below floor2 (a wide margin over rows*u*s0^2): the
+         * rule's walk is reliable when delta^2 is far above it, and the cut
+         * certainly stays at rho0 when the last kept value squared exceeds
+         * delta^2 by more than that error (tail_{rho0-1}^2 =
+         * tail2 + s_{rho0-1}^2 >= delta^2). */
         const double floor2 = 1e4 * rows * 0x1.0p-53 * s0 * s0;
-        trust = !(delta * delta < floor2);
-        if (trust) {
+        const double sl2 = sig[rho0 - 1] * sig[rho0 - 1];
+        if (!(delta * delta < floor2)) {
+            trust = 1;
             rho = tail_rule(sig, rho0, tail2, delta);
             if (rho < 1) rho = 1;
+        } else {
+            trust = (sl2 - floor2 >= delta * delta);
         }
     }
     if (trust) trust = gram_route_accurate(sig, rows, rho);
