@@ -162,3 +162,36 @@ def test_round_edge_cases_match_oracle(case):
     out = to_host(dz)
     assert out.r == ref.r, (out.r, ref.r)
     assert O.rel_diff(out, ref) < 1e-12 + eps
+
+
+@pytest.mark.parametrize("beta,d", [(1.0, 6), (1e-10, 6), (1.0, 10)])
+def test_round_headline_shape_matches_oracle(beta, d):
+    # The headline shape (n = 1024, ranks 64 + 64 -> 128, cap 64, eps 1e-8)
+    # at a length the oracle handles in seconds.  beta = 1: the cap binds at
+    # every bond (the Gram route with the diagonal-ordered factor is taken);
+    # beta = 1e-10: the cut is eps-bound at rank 64.  With the cap binding,
+    # sigma_64 / sigma_63 reaches 0.9995 at some bonds, and the oracle itself
+    # moves by ~1e-12 when its input is perturbed by 1e-15 (a few rounding
+    # errors): the bar is 1e-12 or ten times that measured sensitivity,
+    # whichever is larger.
+    n, rk = 1024, 64
+    sc = 1.0 / np.sqrt(n * rk)
+    ranks = O.uniform_ranks(d, rk)
+    x = O.gauss_tt([n] * d, ranks, 11, sc)
+    y = O.gauss_tt([n] * d, ranks, 12, sc)
+    z = O.axpby(1.0, x, beta, y)
+    cap = [1] + [64] * (d - 1) + [1]
+    tol = 1e-8 * O.norm(z)
+    dz = P.DevTT.from_host(z)
+    P.truncate(dz, tol, cap)
+    out = to_host(dz)
+    ref = O.truncate(z, tol, cap)
+    assert out.r == ref.r, (out.r, ref.r)
+    rel = O.rel_diff(out, ref)
+    rng = np.random.default_rng(0)
+    zp = O.HostTT(z.n, z.r, [c * (1 + 1e-15 * rng.standard_normal(c.size)) for c in z.cores])
+    sens = O.rel_diff(O.truncate(zp, tol, cap), ref)
+    print("d=%d beta=%g: ranks %s, relative difference to the oracle %.3e (oracle's own "
+          "sensitivity to 1e-15 input noise %.3e)" % (d, beta, out.r[:4], rel, sens))
+    assert rel < max(1e-12, 10 * sens)
+    dz.free()
