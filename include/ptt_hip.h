@@ -122,7 +122,8 @@ void ptt_hip_gram(char trans, int c, long len, const double *A, long lda,
  * columns 0) so that A*T has orthonormal columns and A = (A*T)*R.
  * info[0] = dinfo[2] = number of deficient pivots, dinfo[0] =
  * ||Rs||_F*||Rs^-1||_F, dinfo[1] = 2-norm condition of Rs by power
- * iteration when want_kappa (else 0). */
+ * iteration when want_kappa (else 0).  T may be NULL when only R is needed:
+ * then no inverse is formed and dinfo[0] = dinfo[1] = 0. */
 void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny,
                             double *R, double *T, int *info, double *dinfo,
                             int want_kappa);
@@ -158,6 +159,10 @@ void ptt_hip_mlp_adam(float *P, const float *grad, float *m, float *v, float lr,
  * diagonal descending (ties by index), perm (c ints, device) written too;
  * c <= 1024. */
 void ptt_hip_sym_sort_perm(int c, const double *G, double *Gp, int *perm);
+/* Zeroes the smallest-norm columns of L (m x k, ld; k <= 1024) whose squared
+ * norms sum to at most budget * (largest squared column norm); the dropped
+ * sum goes to device *dropped. */
+void ptt_hip_drop_small_cols(int m, int k, double *L, long ld, double budget, double *dropped);
 /* dst[perm[i] + j*ldd] = src[i + j*lds] for i < m, j < n. */
 void ptt_hip_unpermute_rows(int m, int n, const int *perm, const double *src, long lds,
                             double *dst, long ldd);

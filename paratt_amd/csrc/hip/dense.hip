@@ -360,6 +360,20 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         if (tid == 0) S[(long)q * c + q] = rqq;
     }
     __syncthreads();
+    if (T == nullptr) {
+        // factor only: R = Rs * D, no inverse and no condition estimate
+        for (long e = tid; e < (long)c * c; e += nth) {
+            int i = (int)(e % c), k = (int)(e / c);
+            R[e] = (i <= k) ? S[(long)i * c + k] * D[k] : 0.0;
+        }
+        if (tid == 0) {
+            dinfo[0] = 0.0;
+            dinfo[1] = 0.0;
+            dinfo[2] = (double)s_def;
+            info[0] = s_def;
+        }
+        return;
+    }
     // Blocked inverse Ts = Rs^-1 (dead rows and columns zero).  Ts(i,k) for
     // i < k lives at S[k*c + i]; its diagonal in tdg.
     const int nb = (c + CHOL_BLK - 1) / CHOL_BLK;
@@ -918,6 +932,57 @@ __global__ __launch_bounds__(256) void k_sym_sort_perm(int c, const double *__re
     }
 }
 
+// Zeroes the columns of L (m x k, ld) of smallest norm whose squared norms
+// sum to at most budget * max_j ||L_j||^2; the sum dropped is written to
+// *dropped.  Exactly-zero columns take no part in the Jacobi rotations, so
+// columns whose content is below the Gram's own rounding stop costing sweeps.
+__global__ __launch_bounds__(256) void k_drop_small_cols(int m, int k, double *__restrict__ L,
+                                                          long ld, double budget,
+                                                          double *__restrict__ dropped) {
+    __shared__ double sn[SORT_MAXC];
+    __shared__ int so[SORT_MAXC];
+    __shared__ int sdrop[SORT_MAXC];
+    __shared__ double s_max;
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        double s = 0.0;
+        for (int i = 0; i < m; i++) {
+            const double v = L[i + j * ld];
+            s += v * v;
+        }
+        sn[j] = (s == s) ? s : INFINITY;
+        sdrop[j] = 0;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        const double sj = sn[j];
+        int pos = 0;
+        for (int i = 0; i < k; i++) pos += (sn[i] < sj) || (sn[i] == sj && i < j);
+        so[pos] = j;
+    }
+    if (threadIdx.x == 0) {
+        double mx = 0.0;
+        for (int j = 0; j < k; j++) mx = fmax(mx, sn[j]);
+        s_max = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double lim = budget * s_max;
+        double cum = 0.0;
+        for (int q = 0; q < k; q++) {
+            const double v = sn[so[q]];
+            if (!(cum + v <= lim) || v >= s_max) break;
+            cum += v;
+            sdrop[so[q]] = 1;
+        }
+        *dropped = cum;
+    }
+    __syncthreads();
+    for (long e = threadIdx.x; e < (long)m * k; e += blockDim.x) {
+        const long j = e / m, i = e - j * m;
+        if (sdrop[j]) L[i + j * ld] = 0.0;
+    }
+}
+
 // dst[perm[i] + j*ldd] = src[i + j*lds], i < m, j < n.
 __global__ void k_unpermute_rows(int m, int n, const int *__restrict__ perm,
                                  const double *__restrict__ src, long lds,
@@ -959,10 +1024,15 @@ void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny, d
         const size_t fneed = need + 3 * CHOL_BLK * CHOL_BLK;
         // fewer threads for small factors: the steps are barrier-bound
         const int nth = (c <= 16) ? 64 : (c <= 48) ? 256 : NT;
-        PTT_LAUNCH_FB("chol_qr_factor", 2.0 * c * (double)c * c / 3.0, 8.0 * 3.0 * c * (double)c,
-                   k_chol_fast, dim3(1), dim3(nth), fneed * sizeof(double), c,
-                   G, shift, tiny, R, T, info, dinfo, want_kappa);
-    } else if (need <= (size_t)LDS_DOUBLES) {
+        PTT_LAUNCH_FB(T ? "chol_qr_factor" : "chol_factor_only", 2.0 * c * (double)c * c / 3.0,
+                      8.0 * 3.0 * c * (double)c, k_chol_fast, dim3(1), dim3(nth),
+                      fneed * sizeof(double), c, G, shift, tiny, R, T, info, dinfo,
+                      T ? want_kappa : 0);
+        return;
+    }
+    // the large-factor kernel always forms the inverse
+    if (!T) T = (double *)ptt_hip_scratch(PTT_WS_TMP2, sizeof(double) * (size_t)c * c);
+    if (need <= (size_t)LDS_DOUBLES) {
         PTT_LAUNCH("chol_qr_factor", k_chol_qr_factor<true>, dim3(1), dim3(NT),
                    need * sizeof(double), c, G, shift, tiny, R, T, info, dinfo, (double *)nullptr,
                    want_kappa);
@@ -1045,6 +1115,17 @@ void ptt_hip_sym_sort_perm(int c, const double *G, double *Gp, int *perm) {
         exit(1);
     }
     PTT_LAUNCH("sym_sort_perm", k_sym_sort_perm, dim3(1), dim3(256), 0, c, G, Gp, perm);
+}
+
+void ptt_hip_drop_small_cols(int m, int k, double *L, long ld, double budget, double *dropped) {
+    ptt_hip_init();
+    if (m <= 0 || k <= 0) return;
+    if (k > SORT_MAXC) {
+        fprintf(stderr, "ptt_hip_drop_small_cols: k = %d exceeds %d\n", k, SORT_MAXC);
+        exit(1);
+    }
+    PTT_LAUNCH("drop_small_cols", k_drop_small_cols, dim3(1), dim3(256), 0, m, k, L, ld, budget,
+               dropped);
 }
 
 void ptt_hip_unpermute_rows(int m, int n, const int *perm, const double *src, long lds,

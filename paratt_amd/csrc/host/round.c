@@ -34,6 +34,9 @@
 #define KAPPA_ONE_PASS 30.0
 /* Largest accepted estimate of the Gram route's subspace error (rtl_step_fast). */
 #define FAST_EST_MAX 1.0e-13
+/* Budget, in units of rows * u * (largest squared column norm), of the
+ * factor columns the Gram route zeroes before its SVD. */
+#define FAST_DROP 8.0
 #define CHOLQR_MAX_PASSES 6
 #define KAPPA_TWO_PASS 1.0e6
 #define SKETCH_OVERSAMPLE 8
@@ -335,12 +338,14 @@ static int rtl_step_robust(attac_tt *x, int i, const int *maxr, double delta,
  * rows*u*s0 / (s_{rho-1} - s_rho) by the factor s0 / (s_{rho-1} + s_rho).
  * The Gram route is kept only when est is negligible or that factor is at
  * most 2, so its result stays as close to the reference's orthogonal
- * rounding as the problem's own sensitivity allows. */
-static int gram_route_accurate(const double *sig, int rows, int rho) {
+ * rounding as the problem's own sensitivity allows.  `dropped` is the
+ * squared content of the factor columns zeroed before the SVD, added to the
+ * Gram's rounding. */
+static int gram_route_accurate(const double *sig, int rows, int rho, double dropped) {
     const double s0 = sig[0], sl = sig[rho - 1], sn = (rho < rows) ? sig[rho] : 0.0;
     const double gap2 = (sl - sn) * (sl + sn);
     if (!(s0 > 0.0) || !(gap2 > 0.0)) return 0;
-    const double est = rows * 0x1.0p-53 * s0 * s0 / gap2;
+    const double est = (rows * 0x1.0p-53 * s0 * s0 + dropped) / gap2;
     return est <= FAST_EST_MAX || s0 <= 2.0 * (sl + sn);
 }
 
@@ -357,9 +362,12 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
      * so U's rows are put back afterwards. */
     int *perm = (int *)ws->Rtmp;
     ptt_hip_sym_sort_perm(rows, ws->G, ws->T2, perm);
-    ptt_hip_chol_qr_factor(rows, ws->T2, 0.0, tiny_pivot(), ws->R1, ws->T1, ws->iinfo, ws->dinfo,
-                           0);
+    ptt_hip_chol_qr_factor(rows, ws->T2, 0.0, tiny_pivot(), ws->R1, NULL, ws->iinfo, ws->dinfo, 0);
     ptt_hip_transpose(rows, rows, ws->R1, rows, ws->L, rows, 1.0);
+    /* Columns of L whose squared norms together stay below the Gram's own
+     * rounding (FAST_DROP * rows * u * max norm^2) carry no information this
+     * route can resolve: zeroed, they take no part in the rotations. */
+    ptt_hip_drop_small_cols(rows, rows, ws->L, rows, FAST_DROP * rows * 0x1.0p-53, ws->dinfo + 5);
     ptt_hip_svd_jacobi(rows, rows, ws->L, rows, ws->sig, ws->U, rows, NULL, rows, 0.0, delta,
                        maxr[i], ws->iinfo + 1);
     ptt_hip_unpermute_rows(rows, rows, perm, ws->U, rows, ws->V, rows);
@@ -368,9 +376,6 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
     double *hb = (double *)ptt_host_malloc(nb);
     ptt_hip_d2h(hb, ws->sig, nb);
     double *sig = hb;
-    /* Frobenius condition of the scaled factor / rows: a lower bound of its
-     * 2-norm condition (statistics only) */
-    if (i < 64) g_stats.kappa_rtl[i] = hb[rows] / rows;
     const int *io = (const int *)(hb + rows + 8) + 1;
     int numrank = io[0];
     if (i < 64) { g_stats.svd_sweeps[i] = io[2]; g_stats.numrank[i] = numrank; }
@@ -378,6 +383,8 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
     int rho0 = cap < numrank ? cap : numrank;
     if (rho0 < 1) rho0 = 1;
     const double s0 = sig[0];
+    /* statistics: condition of the kept directions */
+    if (i < 64) g_stats.kappa_rtl[i] = sig[rho0 - 1] > 0.0 ? s0 / sig[rho0 - 1] : 0.0;
     if (!(s0 > 0.0)) { free(hb); return -1; }
     int h_rho0 = rho0;
     ptt_hip_h2d(ws->iinfo + 8, &h_rho0, sizeof(int));
@@ -416,7 +423,7 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
             trust = (sl2 - floor2 >= delta * delta);
         }
     }
-    if (trust) trust = gram_route_accurate(sig, rows, rho);
+    if (trust) trust = gram_route_accurate(sig, rows, rho, hb[rows + 5]);
     if (!trust) {
         /* undo: M was overwritten by the residual, restore it */
         ptt_hip_dgemm('N', 'N', rows, (int)w, rho0, 1.0 / nscale[i], ws->US, rows, tmp, rho0, 1.0,
