@@ -585,6 +585,45 @@ __device__ __forceinline__ void jacobi_cs(double al, double be, double ga, doubl
     s = y * t;
 }
 
+// One Jacobi pair step with the lane-group width GG and rows per lane RR
+// known at compile time (m == GG * RR): the LDS offsets of a lane's rows are
+// immediates, so the round spends no VALU work on addresses.  Returns whether
+// the pair rotated; cs / sn / unsettled as in the generic path.
+template <int GG, int RR>
+__device__ __forceinline__ bool jacobi_pair_fixed(double *ca, double *cb, bool valid, double tol,
+                                                  double &cs, double &sn, bool &unsettled) {
+    double xa[RR], xb[RR];
+    double al = 0.0, be = 0.0, ga = 0.0;
+    if (valid) {
+#pragma unroll
+        for (int q = 0; q < RR; q++) {
+            xa[q] = ca[q * GG];
+            xb[q] = cb[q * GG];
+        }
+#pragma unroll
+        for (int q = 0; q < RR; q++) {
+            al += xa[q] * xa[q];
+            be += xb[q] * xb[q];
+            ga += xa[q] * xb[q];
+        }
+    }
+    group_allreduce3_q(al, be, ga, GG);
+    const double nab = sqrt(al) * sqrt(be);
+    const bool rot = valid && fabs(ga) > tol * nab && al > 0.0 && be > 0.0;
+    cs = 1.0;
+    sn = 0.0;
+    if (rot) jacobi_cs(al, be, ga, cs, sn);
+    unsettled = rot && fabs(ga) > 2.0 * tol * nab;
+    if (rot) {
+#pragma unroll
+        for (int q = 0; q < RR; q++) {
+            ca[q * GG] = cs * xa[q] - sn * xb[q];
+            cb[q * GG] = sn * xa[q] + cs * xb[q];
+        }
+    }
+    return rot;
+}
+
 __device__ __forceinline__ void rr_pair(int kp, int t, int p, int &a, int &b) {
     // Round-robin (circle) schedule over kp players, round t in [0, kp-1).
     const int m = kp - 1;
@@ -695,7 +734,12 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
                 // exactly SVD_RV rows per lane (the common m = 8 G case): no
                 // per-row guards, one branch on the pair's validity
                 const bool full = (m == SVD_RV * G);
-                if (full) {
+                // the common shapes get compile-time strides
+                const bool fix16 = (G == 16 && m == 16 * SVD_RV);
+                const bool fix32 = (G == 32 && m == 32 * (SVD_RV / 2));
+                if (fix16 || fix32) {
+                    // loads, dots and rotation inside jacobi_pair_fixed
+                } else if (full) {
                     if (valid) {
 #pragma unroll
                         for (int q = 0; q < SVD_RV; q++) {
@@ -732,19 +776,28 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
                         ga += x * y;
                     }
                 }
+                double cs = 1.0, sn = 0.0;
+                bool rot, unsettled;
+                if (fix16) {
+                    rot = jacobi_pair_fixed<16, SVD_RV>(ca + gl, cb + gl, valid, tol, cs, sn,
+                                                        unsettled);
+                } else if (fix32) {
+                    rot = jacobi_pair_fixed<32, SVD_RV / 2>(ca + gl, cb + gl, valid, tol, cs, sn,
+                                                            unsettled);
+                } else {
                 group_allreduce3_q(al, be, ga, G);
                 const double nab = sqrt(al) * sqrt(be);
-                bool rot = valid && fabs(ga) > tol * nab && al > 0.0 && be > 0.0;
-                double cs = 1.0, sn = 0.0;
+                rot = valid && fabs(ga) > tol * nab && al > 0.0 && be > 0.0;
                 if (rot) jacobi_cs(al, be, ga, cs, sn);
                 // The computed cosine carries rounding up to ~m u even for
                 // exactly orthogonal columns, so rotations start above tol
                 // but only cosines above 2 tol keep the sweeps going;
                 // otherwise noise-triggered rotations never settle.
-                const bool unsettled = rot && fabs(ga) > 2.0 * tol * nab;
+                unsettled = rot && fabs(ga) > 2.0 * tol * nab;
+                }
                 if (LOGROT && p < P && gl == 0)
                     rotlog[((long)sweep * (kp - 1) + t) * P + p] = make_double2(cs, sn);
-                if (rot) {
+                if (rot && !fix16 && !fix32) {
                     if (full) {
 #pragma unroll
                         for (int q = 0; q < SVD_RV; q++) {
@@ -767,6 +820,8 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
                             cb[i] = sn * x + cs * y;
                         }
                     }
+                }
+                if (rot) {
                     if (Vw) {
                         double *va = Vw + (long)a * ldvw, *vb = Vw + (long)b * ldvw;
                         for (int i = gl; i < k; i += G) {
