@@ -522,12 +522,14 @@ __device__ __forceinline__ void group_allreduce3(double &x, double &y, double &z
     }
 }
 
-// DPP move of a double inside a quad / row (dpp_ctrl CTRL).
+// DPP move of a double inside a quad / row (dpp_ctrl CTRL).  Every lane's
+// source is a valid lane for the quad permutations and mirrors used here, so
+// no "old" value needs to be materialised.
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
     int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
 
@@ -608,12 +610,15 @@ __device__ __forceinline__ bool jacobi_pair_fixed(double *ca, double *cb, bool v
         }
     }
     group_allreduce3_q(al, be, ga, GG);
-    const double nab = sqrt(al) * sqrt(be);
-    const bool rot = valid && fabs(ga) > tol * nab && al > 0.0 && be > 0.0;
+    // |cos| > tol compared as ga^2 > tol^2 al be (no square roots; the
+    // columns are scaled to O(1), and products that underflow belong to
+    // columns too small to matter)
+    const double g2 = ga * ga, ab = al * be;
+    const bool rot = valid && g2 > (tol * tol) * ab && al > 0.0 && be > 0.0;
     cs = 1.0;
     sn = 0.0;
     if (rot) jacobi_cs(al, be, ga, cs, sn);
-    unsettled = rot && fabs(ga) > 2.0 * tol * nab;
+    unsettled = rot && g2 > (4.0 * tol * tol) * ab;
     if (rot) {
 #pragma unroll
         for (int q = 0; q < RR; q++) {
