@@ -226,7 +226,7 @@ __global__ __launch_bounds__(NT) void k_chol_qr_factor(int c, const double *__re
 // (diagonal in tdg).  Same outputs and conventions as k_chol_qr_factor.
 constexpr int CHOL_FAST_C = 128;
 constexpr int CHOL_BLK = 32;
-constexpr int CHOL_LDS_DOUBLES = 20400;  // c^2 + 5c + 3*32*32 at c = 128; 159.4 KB
+constexpr int CHOL_LDS_DOUBLES = 20400;  // c(c+1) + 5c + 3*32*32 = 20224 at c = 128; 159.4 KB
 
 __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restrict__ G,
                                                   double shift, double tiny,
@@ -235,8 +235,11 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
     const int nth = blockDim.x;  // 64..1024, a multiple of 64, chosen for c
-    double *S = lds;              // c x c row-major
-    double *D = S + (long)c * c;  // Jacobi scaling
+    // c x c row-major with an odd row stride: column-wise walks (stride ldS)
+    // fall in distinct LDS banks
+    const int ldS = c | 1;
+    double *S = lds;
+    double *D = S + (long)c * ldS;  // Jacobi scaling
     double *tdg = D + c;          // raw pivots, then the diagonal of Ts
     double *dd = tdg + c;         // deficient flags
     double *pv = dd + c;          // power-iteration vectors (2c)
@@ -272,7 +275,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     for (long e = tid; e < (long)c * c; e += nth) {
         int k = (int)(e % c), i = (int)(e / c);
         double di = D[i], dk = D[k];
-        S[e] = (di > 0.0 && dk > 0.0) ? G[k + (long)i * c] / (di * dk) : 0.0;
+        S[(long)i * ldS + k] = (di > 0.0 && dk > 0.0) ? G[k + (long)i * c] / (di * dk) : 0.0;
     }
     // start vector of the power iterations, unit length
     double nrm0 = 0.0;
@@ -289,7 +292,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         for (int i = tid; i < c; i += nth) {
             double rs = 0.0;
             for (int l = 0; l < c; l++)
-                if (l != i) rs += fabs(S[(long)l * c + i]);
+                if (l != i) rs += fabs(S[(long)l * ldS + i]);
             eoff = fmax(eoff, rs);
         }
         for (int o = 32; o > 0; o >>= 1) eoff = fmax(eoff, __shfl_xor(eoff, o));
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
             int i = i0 + row8;
             double acc = 0.0;
             if (i < c)
-                for (int l = sub8; l < c; l += 8) acc += S[(long)i * c + l] * pv[l];
+                for (int l = sub8; l < c; l += 8) acc += S[(long)i * ldS + l] * pv[l];
             acc += __shfl_xor(acc, 1);
             acc += __shfl_xor(acc, 2);
             acc += __shfl_xor(acc, 4);
@@ -335,9 +338,9 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
             const double pq = tdg[q];
             const double rqq = pq > 0.0 ? sqrt(pq) : 0.0;
             const double iq = pq > 0.0 ? 1.0 / rqq : 0.0;
-            for (int k = q + tid; k < c; k += nth) S[(long)q * c + k] = (k == q) ? rqq : S[(long)q * c + k] * iq;
+            for (int k = q + tid; k < c; k += nth) S[(long)q * ldS + k] = (k == q) ? rqq : S[(long)q * ldS + k] * iq;
         }
-        const double p = S[(long)j * c + j] + shift;
+        const double p = S[(long)j * ldS + j] + shift;
         const bool dead = !(D[j] > 0.0) || !(p > tiny);
         if (tid == 0) {
             tdg[j] = dead ? 0.0 : p;
@@ -347,8 +350,8 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         if (!dead) {
             const double ip = 1.0 / p;
             for (int i = j + 1 + hw; i < c; i += nth / 32) {
-                const double sji = S[(long)j * c + i] * ip;
-                for (int k = i + hl; k < c; k += 32) S[(long)i * c + k] -= sji * S[(long)j * c + k];
+                const double sji = S[(long)j * ldS + i] * ip;
+                for (int k = i + hl; k < c; k += 32) S[(long)i * ldS + k] -= sji * S[(long)j * ldS + k];
             }
         }
         __syncthreads();
@@ -357,14 +360,14 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         const int q = c - 1;
         const double pq = tdg[q];
         const double rqq = pq > 0.0 ? sqrt(pq) : 0.0;
-        if (tid == 0) S[(long)q * c + q] = rqq;
+        if (tid == 0) S[(long)q * ldS + q] = rqq;
     }
     __syncthreads();
     if (T == nullptr) {
         // factor only: R = Rs * D, no inverse and no condition estimate
         for (long e = tid; e < (long)c * c; e += nth) {
             int i = (int)(e % c), k = (int)(e / c);
-            R[e] = (i <= k) ? S[(long)i * c + k] * D[k] : 0.0;
+            R[e] = (i <= k) ? S[(long)i * ldS + k] * D[k] : 0.0;
         }
         if (tid == 0) {
             dinfo[0] = 0.0;
@@ -375,12 +378,12 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         return;
     }
     // Blocked inverse Ts = Rs^-1 (dead rows and columns zero).  Ts(i,k) for
-    // i < k lives at S[k*c + i]; its diagonal in tdg.
+    // i < k lives at S[k*ldS + i]; its diagonal in tdg.
     const int nb = (c + CHOL_BLK - 1) / CHOL_BLK;
-    for (int j = tid; j < c; j += nth) tdg[j] = (dd[j] != 0.0) ? 0.0 : 1.0 / S[(long)j * c + j];
+    for (int j = tid; j < c; j += nth) tdg[j] = (dd[j] != 0.0) ? 0.0 : 1.0 / S[(long)j * ldS + j];
     __syncthreads();
     auto tget = [&](int i, int k) -> double {  // Ts(i,k), i <= k
-        return (i == k) ? tdg[i] : S[(long)k * c + i];
+        return (i == k) ? tdg[i] : S[(long)k * ldS + i];
     };
     // diagonal blocks, all in parallel: column jj of every block per step
     for (int jj = 1; jj < CHOL_BLK && jj < c; jj++) {
@@ -392,12 +395,12 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
             double acc = 0.0;
             const bool act = (j < c) && (il < jj);
             if (act && dd[j] == 0.0)
-                for (int l = i + sub8; l < j; l += 8) acc += tget(i, l) * S[(long)l * c + j];
+                for (int l = i + sub8; l < j; l += 8) acc += tget(i, l) * S[(long)l * ldS + j];
             acc += __shfl_xor(acc, 1);
             acc += __shfl_xor(acc, 2);
             acc += __shfl_xor(acc, 4);
             // writes go to row j of the lower triangle; this step reads rows < j
-            if (act && sub8 == 0) S[(long)j * c + i] = -acc * tdg[j];
+            if (act && sub8 == 0) S[(long)j * ldS + i] = -acc * tdg[j];
         }
         __syncthreads();
     }
@@ -412,7 +415,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
             double acc = 0.0;
             if (ia < c && jb < c) {
                 const int l0 = (bi + 1) * CHOL_BLK;
-                for (int n = l0; n <= jb; n++) acc += S[(long)ia * c + n] * tget(n, jb);
+                for (int n = l0; n <= jb; n++) acc += S[(long)ia * ldS + n] * tget(n, jb);
             }
             Y[e] = acc;
         }
@@ -427,7 +430,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
                 const int mend = min((bi + 1) * CHOL_BLK, c);
                 for (int m = ia; m < mend; m++)
                     acc += tget(ia, m) * Y[pi * CHOL_BLK * CHOL_BLK + (m - bi * CHOL_BLK) * CHOL_BLK + b];
-                S[(long)jb * c + ia] = -acc;
+                S[(long)jb * ldS + ia] = -acc;
             }
         }
         __syncthreads();
@@ -436,7 +439,7 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     double fr = 0.0, ft = 0.0;
     for (long e = tid; e < (long)c * c; e += nth) {
         int i = (int)(e % c), k = (int)(e / c);
-        double rv = (i <= k) ? S[(long)i * c + k] : 0.0;
+        double rv = (i <= k) ? S[(long)i * ldS + k] : 0.0;
         double tv = (i <= k) ? tget(i, k) : 0.0;
         fr += rv * rv;
         ft += tv * tv;
@@ -1081,7 +1084,7 @@ void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny, d
     allow_big_lds();
     size_t need = (size_t)c * c + 5 * (size_t)c;
     if (c <= CHOL_FAST_C) {
-        const size_t fneed = need + 3 * CHOL_BLK * CHOL_BLK;
+        const size_t fneed = (size_t)c * (c | 1) + 5 * (size_t)c + 3 * CHOL_BLK * CHOL_BLK;
         // fewer threads for small factors: the steps are barrier-bound
         const int nth = (c <= 16) ? 64 : (c <= 48) ? 256 : NT;
         PTT_LAUNCH_FB(T ? "chol_qr_factor" : "chol_factor_only", 2.0 * c * (double)c * c / 3.0,
