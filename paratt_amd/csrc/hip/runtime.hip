@@ -43,6 +43,19 @@ static void host_mode_copy(void *dst, const void *src, size_t bytes) {
     memcpy(dst, src, bytes);
 }
 
+// Small transfers between device memory and the caller's (pageable) host
+// memory go through coherent pinned staging buffers moved by k_copy_bytes:
+// a one-wave kernel instead of the runtime's staged blit copy.
+constexpr size_t STAGE_BYTES = 64 * 1024;
+static unsigned char *g_stage_d2h = nullptr, *g_stage_h2d = nullptr;
+
+static void launch_copy_bytes(void *dst, const void *src, size_t bytes) {
+    unsigned blocks = (unsigned)((bytes / 8 + 255) / 256);
+    blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
+    PTT_LAUNCH("copy_bytes", k_copy_bytes, dim3(blocks), dim3(256), 0, (unsigned char *)dst,
+               (const unsigned char *)src, bytes);
+}
+
 void fatal_hip(hipError_t e, const char *what, const char *file, int line) {
     fprintf(stderr, "paratt_amd: HIP error '%s' (%d) in %s at %s:%d\n",
             hipGetErrorString(e), (int)e, what, file, line);
@@ -130,6 +143,8 @@ void ptt_hip_init(void) {
     HIPCHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     const char *hv = getenv("PARATT_AMD_HOST_VISIBLE");
     g_host_visible = hv != nullptr && hv[0] != '\0' && hv[0] != '0';
+    HIPCHECK(hipHostMalloc((void **)&g_stage_d2h, STAGE_BYTES, hipHostMallocCoherent));
+    HIPCHECK(hipHostMalloc((void **)&g_stage_h2d, STAGE_BYTES, hipHostMallocCoherent));
     g_init = true;
 }
 
@@ -165,6 +180,13 @@ void ptt_hip_h2d(void *dst, const void *src, size_t bytes) {
     ptt_hip_init();
     if (!bytes) return;
     if (g_host_visible) return host_mode_copy(dst, src, bytes);
+    if (bytes <= STAGE_BYTES) {
+        // the previous small upload has completed (every h2d ends synchronised)
+        memcpy(g_stage_h2d, src, bytes);
+        launch_copy_bytes(dst, g_stage_h2d, bytes);
+        HIPCHECK(hipStreamSynchronize(g_stream));
+        return;
+    }
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
     HIPCHECK(hipStreamSynchronize(g_stream));
 }
@@ -173,6 +195,12 @@ void ptt_hip_d2h(void *dst, const void *src, size_t bytes) {
     ptt_hip_init();
     if (!bytes) return;
     if (g_host_visible) return host_mode_copy(dst, src, bytes);
+    if (bytes <= STAGE_BYTES) {
+        launch_copy_bytes(g_stage_d2h, src, bytes);
+        HIPCHECK(hipStreamSynchronize(g_stream));
+        memcpy(dst, g_stage_d2h, bytes);
+        return;
+    }
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
     HIPCHECK(hipStreamSynchronize(g_stream));
 }
@@ -181,10 +209,7 @@ void ptt_hip_d2d(void *dst, const void *src, size_t bytes) {
     ptt_hip_init();
     if (!bytes || dst == src) return;
     if (g_host_visible) {
-        unsigned blocks = (unsigned)((bytes / 8 + 255) / 256);
-        blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
-        PTT_LAUNCH("copy_bytes", k_copy_bytes, dim3(blocks), dim3(256), 0, (unsigned char *)dst,
-                   (const unsigned char *)src, bytes);
+        launch_copy_bytes(dst, src, bytes);
         return;
     }
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
