@@ -239,7 +239,12 @@ void gemm_impl(char ta, char tb, int M, int N, int K, double alpha, const double
         }
         return;
     }
-    const int BM = (M <= 64) ? 64 : 128, BN = (N <= 64) ? 64 : 128;
+    // row Gram products (A A^T) of up to 128 rows use 64 x 64 tiles: four
+    // times the workgroups of one 128 x 128 tile per split, so every CU
+    // holds several and hides the streaming latency of its slab (measured
+    // faster for A A^T, slower for A^T A, whose slabs are column panels)
+    const bool small_tiles = force_split && M <= 128 && N <= 128 && (tb == 'T' || tb == 't');
+    const int BM = (M <= 64 || small_tiles) ? 64 : 128, BN = (N <= 64 || small_tiles) ? 64 : 128;
     const unsigned gm = cdiv(M, BM), gn = cdiv(N, BN);
     int nsplit = 1;
     long tiles = (long)gm * gn;
