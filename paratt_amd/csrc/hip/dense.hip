@@ -715,6 +715,11 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
     const int P = kp / 2;
     int G = 64;
     while (G > 1 && G * P > nth) G >>= 1;
+    // about SVD_RV rows per lane: wider groups only add redundant reduction
+    // and rotation-parameter work (idle groups skip the round below)
+    int Gr = 1;
+    while (Gr < 64 && Gr * SVD_RV < m) Gr <<= 1;
+    if (Gr < G) G = Gr;
     const int ngroups = nth / G;
     const int g = tid / G, gl = tid - g * G;
     // Rotate only when the pair's cosine exceeds the rounding noise of an
@@ -730,6 +735,9 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
         for (int t = 0; t < kp - 1; t++) {
             for (int p0 = 0; p0 < P; p0 += ngroups) {
                 int p = p0 + g;
+                // groups without a pair this round (all lanes of a group agree;
+                // nothing below reaches across groups)
+                if (p >= P) continue;
                 int a = -1, b = -1;
                 if (p < P) rr_pair_fast(kp, t, p, a, b);
                 bool valid = (p < P) && a < knz && b < knz;
