@@ -48,6 +48,9 @@ size_t ptt_hip_scratch_cap(int slot);
 /* ---- profiling: HIP events around launches, per kernel name ---- */
 void ptt_hip_prof_enable(int on);
 void ptt_hip_prof_reset(void);
+/* Record only launches named `name` (NULL or "" records every launch): keeps
+ * the event overhead out of a timed region while one kernel is measured. */
+void ptt_hip_prof_only(const char *name);
 /* Returns 1 and fills count / total_ms if `name` was recorded (syncs). */
 int ptt_hip_prof_get(const char *name, int *count, double *total_ms);
 /* Algorithmic flops and bytes credited to `name` (syncs). */

@@ -256,6 +256,13 @@ def prof_enable(on=True):
     lib().ptt_hip_prof_enable(1 if on else 0)
 
 
+def prof_only(name=None):
+    """Record only launches of kernel `name` (None: all)."""
+    L = lib()
+    L.ptt_hip_prof_only.argtypes = [C.c_char_p]
+    L.ptt_hip_prof_only(name.encode() if name else None)
+
+
 def prof_reset():
     lib().ptt_hip_prof_reset()
 

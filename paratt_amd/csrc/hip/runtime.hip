@@ -75,6 +75,7 @@ struct ProfRec {
     double flops, bytes;
 };
 static bool g_prof = false;
+static std::string g_prof_only;  // when set, only launches of this name are recorded
 static std::vector<ProfRec> g_pending;
 static std::vector<hipEvent_t> g_free_events;
 struct ProfAcc {
@@ -112,7 +113,8 @@ static void resolve_pending() {
 }
 
 ProfScope::ProfScope(const char *name, double f, double b)
-    : active(g_prof), nm(name), flops(f), bytes(b) {
+    : active(g_prof && (g_prof_only.empty() || g_prof_only == name)), nm(name), flops(f),
+      bytes(b) {
     if (!active) return;
     a = get_event();
     HIPCHECK(hipEventRecord(a, g_stream));
@@ -260,6 +262,12 @@ void ptt_hip_prof_enable(int on) {
     ptt_hip_init();
     if (!on) resolve_pending();
     g_prof = on != 0;
+}
+
+void ptt_hip_prof_only(const char *name) {
+    ptt_hip_init();
+    resolve_pending();
+    g_prof_only = name ? name : "";
 }
 
 void ptt_hip_prof_reset(void) {
