@@ -219,22 +219,28 @@ __global__ __launch_bounds__(NT) void k_chol_qr_factor(int c, const double *__re
     (void)s_scal;
 }
 
-// Low-latency variant for c <= 128 (the whole problem in LDS): one barrier
-// per elimination step (the scaling of pivot row j is deferred to step j+1,
-// which does not read that row), and a blocked triangular inverse whose
-// strictly upper part is kept transposed in the unused lower triangle of S
-// (diagonal in tdg).  Same outputs and conventions as k_chol_qr_factor.
+// Low-latency variant for c <= CMAX <= 128 (the whole problem in LDS, the
+// trailing matrix of the elimination in registers, one barrier per
+// elimination step).  With WT the elimination also carries the identity
+// ([S | I] -> [U | L^-1], S = L U, U = diag(p) L^T), which gives the inverse
+// of the scaled factor without a separate triangular inversion:
+// Rs = diag(sqrt p) L^T, so Ts = Rs^-1 = L^-T diag(1/sqrt p).  A dead pivot
+// skips its step, which leaves row and column j of Ts zero.  Ts(i,k) for
+// i < k is kept transposed in the lower triangle of S, its diagonal in tdg.
+// NTH threads (64, 256 or 512, chosen for c).  Same outputs and conventions
+// as k_chol_qr_factor.
 constexpr int CHOL_FAST_C = 128;
 constexpr int CHOL_BLK = 32;
 constexpr int CHOL_LDS_DOUBLES = 20400;  // c(c+1) + 5c + 3*32*32 = 20224 at c = 128; 159.4 KB
 
-__global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restrict__ G,
-                                                  double shift, double tiny,
-                                                  double *__restrict__ R, double *__restrict__ T,
-                                                  int *info, double *dinfo, int want_kappa) {
+template <int NTH, int CMAX, bool WT>
+__global__ __launch_bounds__(NTH) void k_chol_fast(int c, const double *__restrict__ G,
+                                                   double shift, double tiny,
+                                                   double *__restrict__ R, double *__restrict__ T,
+                                                   int *info, double *dinfo, int want_kappa) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
-    const int nth = blockDim.x;  // 64..1024, a multiple of 64, chosen for c
+    constexpr int nth = NTH;
     // c x c row-major with an odd row stride: column-wise walks (stride ldS)
     // fall in distinct LDS banks
     const int ldS = c | 1;
@@ -270,12 +276,16 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         double g = G[j + (long)j * c];
         D[j] = g > 0.0 ? sqrt(g) : 0.0;
     }
-    if (tid == 0) s_def = 0;
+    // half-wave hw of 32 lanes (lane hl); H half-waves in the workgroup
+    constexpr int H = NTH / 32;
+    const int hw = tid >> 5, hl = tid & 31;
     __syncthreads();
-    for (long e = tid; e < (long)c * c; e += nth) {
-        int k = (int)(e % c), i = (int)(e / c);
-        double di = D[i], dk = D[k];
-        S[(long)i * ldS + k] = (di > 0.0 && dk > 0.0) ? G[k + (long)i * c] / (di * dk) : 0.0;
+    for (int i = hw; i < c; i += H) {
+        const double di = D[i];
+        for (int k = hl; k < c; k += 32) {
+            const double dk = D[k];
+            S[i * ldS + k] = (di > 0.0 && dk > 0.0) ? G[k + (long)i * c] / (di * dk) : 0.0;
+        }
     }
     // start vector of the power iterations, unit length
     double nrm0 = 0.0;
@@ -288,11 +298,18 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     // 1/2 then kappa_2(Gram) <= (1+e)/(1-e) and the power iterations are
     // skipped (the bound is reported instead of the estimate).
     double eoff = 0.0, dummy0 = 0.0;
+    const int row8 = tid >> 3, sub8 = tid & 7;
     if (want_kappa) {
-        for (int i = tid; i < c; i += nth) {
+        // row sums of the symmetric S, 8 lanes per row
+        for (int i0 = 0; i0 < c; i0 += nth / 8) {
+            const int i = i0 + row8;
             double rs = 0.0;
-            for (int l = 0; l < c; l++)
-                if (l != i) rs += fabs(S[(long)l * ldS + i]);
+            if (i < c)
+                for (int l = sub8; l < c; l += 8)
+                    if (l != i) rs += fabs(S[(long)i * ldS + l]);
+            rs += __shfl_xor(rs, 1);
+            rs += __shfl_xor(rs, 2);
+            rs += __shfl_xor(rs, 4);
             eoff = fmax(eoff, rs);
         }
         for (int o = 32; o > 0; o >>= 1) eoff = fmax(eoff, __shfl_xor(eoff, o));
@@ -306,7 +323,6 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
     (void)dummy0;
     const bool gersh = want_kappa && eoff < 0.5;
     const int PIT = (want_kappa && !gersh) ? 6 : 0;
-    const int row8 = tid >> 3, sub8 = tid & 7;
     double lmax = 0.0;
     for (int it = 0; it < PIT; it++) {
         // y = S x with x of unit length: lambda ~ ||y||
@@ -329,45 +345,117 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         for (int i = tid; i < c; i += nth) pv[i] = pv[c + i] * inv;
         __syncthreads();
     }
-    // Cholesky, upper triangle of S, one barrier per step.
-    const int hw = tid >> 5, hl = tid & 31;  // 32 half-waves of 32 lanes
-    for (int j = 0; j < c; j++) {
-        if (j > 0) {
-            // deferred: finish row j-1
-            const int q = j - 1;
-            const double pq = tdg[q];
-            const double rqq = pq > 0.0 ? sqrt(pq) : 0.0;
-            const double iq = pq > 0.0 ? 1.0 / rqq : 0.0;
-            for (int k = q + tid; k < c; k += nth) S[(long)q * ldS + k] = (k == q) ? rqq : S[(long)q * ldS + k] * iq;
+    // Cholesky, upper triangle.  Thread (hw, hl) holds S[hw + H s][hl + 32 t]
+    // in registers; at step j the half-wave owning row j publishes it through
+    // a double-buffered LDS row (one barrier per step: the buffer written at
+    // step j + 2 was last read before the barrier of step j + 1).  Rows are
+    // scaled by 1/sqrt(pivot) once all pivots are known.
+    constexpr int SN = (CMAX + H - 1) / H, TN = (CMAX + 31) / 32, RB = 32 * TN;
+    constexpr int NP = WT ? 2 : 1;  // row parts published per step
+    static_assert(H * SN <= RB && 2 * NP * RB <= 3 * CHOL_BLK * CHOL_BLK, "row buffer layout");
+    double a[SN][TN];
+    double e[WT ? SN : 1][WT ? TN : 1];  // rows of L^-1 (identity at the start)
+    if (WT) {
+#pragma unroll
+        for (int s2 = 0; s2 < SN; s2++)
+#pragma unroll
+            for (int t = 0; t < TN; t++) e[s2][t] = (hw + H * s2 == hl + 32 * t) ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < SN; s2++)
+#pragma unroll
+        for (int t = 0; t < TN; t++) {
+            const int i = hw + H * s2, k = hl + 32 * t;
+            a[s2][t] = (i < c && k < c) ? S[(long)i * ldS + k] : 0.0;
         }
-        const double p = S[(long)j * ldS + j] + shift;
-        const bool dead = !(D[j] > 0.0) || !(p > tiny);
+    // Y is free until the outputs: two rows of NP * RB doubles.  Every read of a
+    // step is issued at once (indices < RB, values outside the active
+    // triangle are masked), and the deficiency count is kept in a register
+    // (the pivot test is uniform across the workgroup).
+    double *rowbuf = Y;
+    int ndef = 0;
+    for (int j = 0; j < c; j++) {
+        double *rb = rowbuf + (j & 1) * NP * RB;
+        if (hw == j % H) {
+            const int sj = j / H;
+#pragma unroll
+            for (int s2 = 0; s2 < SN; s2++)
+                if (s2 == sj) {
+#pragma unroll
+                    for (int t = 0; t < TN; t++) {
+                        rb[hl + 32 * t] = a[s2][t];
+                        if (WT) rb[RB + hl + 32 * t] = e[WT ? s2 : 0][WT ? t : 0];
+                    }
+                }
+        }
+        __syncthreads();
+        const double pj = rb[j], dj = D[j];
+        double rk[TN], ri[SN], re[TN];
+#pragma unroll
+        for (int t = 0; t < TN; t++) {
+            rk[t] = rb[hl + 32 * t];
+            re[t] = WT ? rb[RB + hl + 32 * t] : 0.0;
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < SN; s2++) ri[s2] = rb[hw + H * s2];
+        const double p = pj + shift;
+        const bool dead = !(dj > 0.0) || !(p > tiny);
+        ndef += dead ? 1 : 0;
         if (tid == 0) {
             tdg[j] = dead ? 0.0 : p;
             dd[j] = dead ? 1.0 : 0.0;
-            if (dead) s_def += 1;
         }
         if (!dead) {
+            // Only rows below j change (their multiplier is zero otherwise).
+            // Columns are not masked: entries left of the diagonal or beyond
+            // c never feed the upper triangle, the published row holds finite
+            // values everywhere (zeros beyond c), and row j of L^-1 is
+            // exactly zero beyond column j.
+            // Register blocks with nothing to do are skipped on uniform
+            // tests: row blocks whose rows are all <= j, column blocks of U
+            // entirely at or left of column j, and column blocks of L^-1
+            // entirely right of column j.
             const double ip = 1.0 / p;
-            for (int i = j + 1 + hw; i < c; i += nth / 32) {
-                const double sji = S[(long)j * ldS + i] * ip;
-                for (int k = i + hl; k < c; k += 32) S[(long)i * ldS + k] -= sji * S[(long)j * ldS + k];
+#pragma unroll
+            for (int s2 = 0; s2 < SN; s2++) {
+                if (H * s2 + H - 1 <= j) continue;
+                const int i = hw + H * s2;
+                const double sji = (i > j) ? ri[s2] * ip : 0.0;
+#pragma unroll
+                for (int t = 0; t < TN; t++) {
+                    if (32 * t + 31 > j) a[s2][t] = a[s2][t] - sji * rk[t];
+                    if (WT && 32 * t <= j)
+                        e[WT ? s2 : 0][WT ? t : 0] = e[WT ? s2 : 0][WT ? t : 0] - sji * re[t];
+                }
             }
         }
-        __syncthreads();
     }
-    {
-        const int q = c - 1;
-        const double pq = tdg[q];
-        const double rqq = pq > 0.0 ? sqrt(pq) : 0.0;
-        if (tid == 0) S[(long)q * ldS + q] = rqq;
-    }
+    if (tid == 0) s_def = ndef;
+    __syncthreads();  // every pivot is in tdg
+#pragma unroll
+    for (int s2 = 0; s2 < SN; s2++)
+#pragma unroll
+        for (int t = 0; t < TN; t++) {
+            const int i = hw + H * s2, k = hl + 32 * t;
+            if (i < c && k >= i && k < c) {
+                const double pq = tdg[i];
+                const double rqq = pq > 0.0 ? sqrt(pq) : 0.0;
+                const double iq = pq > 0.0 ? 1.0 / rqq : 0.0;
+                S[(long)i * ldS + k] = (k == i) ? rqq : a[s2][t] * iq;
+            }
+            if (WT && i < c && k < i) {
+                // Ts(k, i) = L^-1(i, k) / sqrt(p_i), stored at S[i * ldS + k]
+                const double pq = tdg[i];
+                const double iq = pq > 0.0 ? 1.0 / sqrt(pq) : 0.0;
+                S[(long)i * ldS + k] = e[WT ? s2 : 0][WT ? t : 0] * iq;
+            }
+        }
     __syncthreads();
-    if (T == nullptr) {
+    if (!WT) {
         // factor only: R = Rs * D, no inverse and no condition estimate
-        for (long e = tid; e < (long)c * c; e += nth) {
-            int i = (int)(e % c), k = (int)(e / c);
-            R[e] = (i <= k) ? S[(long)i * ldS + k] * D[k] : 0.0;
+        for (int k = hw; k < c; k += H) {
+            const double dk = D[k];
+            for (int i = hl; i < c; i += 32) R[i + (long)k * c] = (i <= k) ? S[i * ldS + k] * dk : 0.0;
         }
         if (tid == 0) {
             dinfo[0] = 0.0;
@@ -377,74 +465,24 @@ __global__ __launch_bounds__(NT) void k_chol_fast(int c, const double *__restric
         }
         return;
     }
-    // Blocked inverse Ts = Rs^-1 (dead rows and columns zero).  Ts(i,k) for
-    // i < k lives at S[k*ldS + i]; its diagonal in tdg.
-    const int nb = (c + CHOL_BLK - 1) / CHOL_BLK;
+    // diagonal of Ts (dead rows and columns are zero)
     for (int j = tid; j < c; j += nth) tdg[j] = (dd[j] != 0.0) ? 0.0 : 1.0 / S[(long)j * ldS + j];
     __syncthreads();
     auto tget = [&](int i, int k) -> double {  // Ts(i,k), i <= k
         return (i == k) ? tdg[i] : S[(long)k * ldS + i];
     };
-    // diagonal blocks, all in parallel: column jj of every block per step
-    for (int jj = 1; jj < CHOL_BLK && jj < c; jj++) {
-        // rows i of block q with i < j = q*32 + jj; 8 lanes per (q, i)
-        for (int rr = row8; rr < nb * CHOL_BLK; rr += nth / 8) {
-            const int q = rr >> 5, il = rr & 31;
-            const int j = q * CHOL_BLK + jj;
-            const int i = q * CHOL_BLK + il;
-            double acc = 0.0;
-            const bool act = (j < c) && (il < jj);
-            if (act && dd[j] == 0.0)
-                for (int l = i + sub8; l < j; l += 8) acc += tget(i, l) * S[(long)l * ldS + j];
-            acc += __shfl_xor(acc, 1);
-            acc += __shfl_xor(acc, 2);
-            acc += __shfl_xor(acc, 4);
-            // writes go to row j of the lower triangle; this step reads rows < j
-            if (act && sub8 == 0) S[(long)j * ldS + i] = -acc * tdg[j];
-        }
-        __syncthreads();
-    }
-    // off-diagonal blocks by distance: Ts_ij = -Ts_ii * sum_{l=i+1..j} Rs_il Ts_lj
-    for (int sd = 1; sd < nb; sd++) {
-        const int npair = nb - sd;
-        for (int e = tid; e < npair * CHOL_BLK * CHOL_BLK; e += nth) {
-            const int pi = e / (CHOL_BLK * CHOL_BLK), r = e - pi * CHOL_BLK * CHOL_BLK;
-            const int a = r / CHOL_BLK, b = r - a * CHOL_BLK;
-            const int bi = pi, bj = pi + sd;
-            const int ia = bi * CHOL_BLK + a, jb = bj * CHOL_BLK + b;
-            double acc = 0.0;
-            if (ia < c && jb < c) {
-                const int l0 = (bi + 1) * CHOL_BLK;
-                for (int n = l0; n <= jb; n++) acc += S[(long)ia * ldS + n] * tget(n, jb);
-            }
-            Y[e] = acc;
-        }
-        __syncthreads();
-        for (int e = tid; e < npair * CHOL_BLK * CHOL_BLK; e += nth) {
-            const int pi = e / (CHOL_BLK * CHOL_BLK), r = e - pi * CHOL_BLK * CHOL_BLK;
-            const int a = r / CHOL_BLK, b = r - a * CHOL_BLK;
-            const int bi = pi, bj = pi + sd;
-            const int ia = bi * CHOL_BLK + a, jb = bj * CHOL_BLK + b;
-            if (ia < c && jb < c) {
-                double acc = 0.0;
-                const int mend = min((bi + 1) * CHOL_BLK, c);
-                for (int m = ia; m < mend; m++)
-                    acc += tget(ia, m) * Y[pi * CHOL_BLK * CHOL_BLK + (m - bi * CHOL_BLK) * CHOL_BLK + b];
-                S[(long)jb * ldS + ia] = -acc;
-            }
-        }
-        __syncthreads();
-    }
     // outputs and Frobenius norms
     double fr = 0.0, ft = 0.0;
-    for (long e = tid; e < (long)c * c; e += nth) {
-        int i = (int)(e % c), k = (int)(e / c);
-        double rv = (i <= k) ? S[(long)i * ldS + k] : 0.0;
-        double tv = (i <= k) ? tget(i, k) : 0.0;
-        fr += rv * rv;
-        ft += tv * tv;
-        R[e] = rv * D[k];
-        T[e] = (D[i] > 0.0) ? tv / D[i] : 0.0;
+    for (int k = hw; k < c; k += H) {
+        const double dk = D[k];
+        for (int i = hl; i < c; i += 32) {
+            const double rv = (i <= k) ? S[i * ldS + k] : 0.0;
+            const double tv = (i <= k) ? tget(i, k) : 0.0;
+            fr += rv * rv;
+            ft += tv * tv;
+            R[i + (long)k * c] = rv * dk;
+            T[i + (long)k * c] = (D[i] > 0.0) ? tv / D[i] : 0.0;
+        }
     }
     block_sum2(fr, ft);
     // lambda_max(Ts Ts^T) by power iteration
@@ -1076,7 +1114,12 @@ void allow_big_lds() {
     static bool done = false;
     if (done) return;
     set_lds_attr(k_chol_qr_factor<true>, LDS_DOUBLES);
-    set_lds_attr(k_chol_fast, CHOL_LDS_DOUBLES);
+    set_lds_attr(k_chol_fast<64, 16, false>, CHOL_LDS_DOUBLES);
+    set_lds_attr(k_chol_fast<256, 48, false>, CHOL_LDS_DOUBLES);
+    set_lds_attr(k_chol_fast<512, CHOL_FAST_C, false>, CHOL_LDS_DOUBLES);
+    set_lds_attr(k_chol_fast<64, 16, true>, CHOL_LDS_DOUBLES);
+    set_lds_attr(k_chol_fast<256, 48, true>, CHOL_LDS_DOUBLES);
+    set_lds_attr(k_chol_fast<512, CHOL_FAST_C, true>, CHOL_LDS_DOUBLES);
     set_lds_attr(k_svd_jacobi<true, true, false>, SVD_LDS_DOUBLES);
     set_lds_attr(k_svd_jacobi<true, false, true>, SVD_LDS_DOUBLES);
     done = true;
@@ -1094,11 +1137,25 @@ void ptt_hip_chol_qr_factor(int c, const double *G, double shift, double tiny, d
     if (c <= CHOL_FAST_C) {
         const size_t fneed = (size_t)c * (c | 1) + 5 * (size_t)c + 3 * CHOL_BLK * CHOL_BLK;
         // fewer threads for small factors: the steps are barrier-bound
-        const int nth = (c <= 16) ? 64 : (c <= 48) ? 256 : NT;
-        PTT_LAUNCH_FB(T ? "chol_qr_factor" : "chol_factor_only", 2.0 * c * (double)c * c / 3.0,
-                      8.0 * 3.0 * c * (double)c, k_chol_fast, dim3(1), dim3(nth),
-                      fneed * sizeof(double), c, G, shift, tiny, R, T, info, dinfo,
-                      T ? want_kappa : 0);
+        const char *nm = T ? "chol_qr_factor" : "chol_factor_only";
+        const double fl = 2.0 * c * (double)c * c / 3.0, by = 8.0 * 3.0 * c * (double)c;
+        const int wk = T ? want_kappa : 0;
+#define PTT_CHOL_FAST(NTH_, CMAX_)                                                         \
+    do {                                                                                    \
+        if (T)                                                                              \
+            PTT_LAUNCH_FB(nm, fl, by, (k_chol_fast<NTH_, CMAX_, true>), dim3(1), dim3(NTH_),  \
+                          fneed * sizeof(double), c, G, shift, tiny, R, T, info, dinfo, wk);  \
+        else                                                                                \
+            PTT_LAUNCH_FB(nm, fl, by, (k_chol_fast<NTH_, CMAX_, false>), dim3(1), dim3(NTH_), \
+                          fneed * sizeof(double), c, G, shift, tiny, R, T, info, dinfo, wk);  \
+    } while (0)
+        if (c <= 16)
+            PTT_CHOL_FAST(64, 16);
+        else if (c <= 48)
+            PTT_CHOL_FAST(256, 48);
+        else
+            PTT_CHOL_FAST(512, CHOL_FAST_C);
+#undef PTT_CHOL_FAST
         return;
     }
     // the large-factor kernel always forms the inverse
