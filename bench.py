@@ -26,6 +26,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
+# HBM bytes per launch of the kernels that can dominate this workload, from
+# two rocprofv3 PMC runs of this benchmark (FETCH_SIZE x 2 + WRITE_SIZE x 1,
+# factors from a plain copy of known size), with the summary they come from.
+PMC_TRAFFIC = {
+    "svd_jacobi": (292864, "profiles/svd_jacobi_d40_n1024_r128_pmc_traffic.md"),
+}
+
 
 def cpu_baseline(d_sample, n, rank, eps, cap, d_full):
     """Reference CPU path on a bounded sample (d_sample cores of the same
@@ -155,9 +162,12 @@ def main():
     dom = [k for k in collect() if k["name"] == dom_name][0]
     if dom["flops"] > 0:
         ach = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+        tr = PMC_TRAFFIC.get(dom["name"]) if (a.d, a.n, a.rank, a.cap) == (40, 1024, 64, 64) else None
         roof = {"bound": "mfma", "kernel": dom["name"], "achieved": round(ach, 2),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4),
-                "traffic": None,
+                "traffic": tr[0] if tr else None,
+                "traffic_unit": "bytes per launch (HBM, rocprofv3 PMC)",
+                "traffic_source": tr[1] if tr else None,
                 "avg_launch_us": round(1000.0 * dom["ms"] / max(dom["launches"], 1), 2),
                 "flops_per_launch": dom["flops"] / max(dom["launches"], 1)}
     else:

@@ -244,7 +244,15 @@ void gemm_impl(char ta, char tb, int M, int N, int K, double alpha, const double
     // holds several and hides the streaming latency of its slab (measured
     // faster for A A^T, slower for A^T A, whose slabs are column panels)
     const bool small_tiles = force_split && M <= 128 && N <= 128 && (tb == 'T' || tb == 't');
-    const int BM = (M <= 64 || small_tiles) ? 64 : 128, BN = (N <= 64 || small_tiles) ? 64 : 128;
+    int BM = (M <= 64 || small_tiles) ? 64 : 128, BN = (N <= 64 || small_tiles) ? 64 : 128;
+    // A 128 x 128 tile holds 360 registers per lane (one workgroup per CU);
+    // 128 x 64 and 64 x 128 tiles fit two, which overlaps one tile's loads
+    // and stores with another's MFMA.  The tile still spans the whole short
+    // dimension, which the in-place products (R * core, core * T) rely on.
+    if (!force_split && BM == 128 && BN == 128) {
+        if (M <= 128 && N >= 4096) BN = 64;
+        else if (N <= 128 && M >= 4096) BM = 64;
+    }
     const unsigned gm = cdiv(M, BM), gn = cdiv(N, BN);
     int nsplit = 1;
     long tiles = (long)gm * gn;
