@@ -33,11 +33,24 @@ void ptt_hip_copy(void *dst, const void *src, size_t bytes);
  * pinned host memory that the GPU accesses through the same address, so
  * host code may read and write cores directly between calls. */
 int ptt_hip_host_visible(void);
-void ptt_hip_sync(void);
+void ptt_hip_sync(void);  /* waits for the main and the side stream */
+/* A second stream for work off the critical path.  side_begin orders the side
+ * stream after everything queued so far on the main stream and sends later
+ * launches to it; side_end(slot) records the side stream's progress in event
+ * `slot` (0 or 1) and returns launches to the main stream; side_wait(slot)
+ * makes the main stream wait for that event; side_join makes the main stream
+ * wait for all side work.  Only kernel launches (no copies) belong between
+ * side_begin and side_end.  side_enabled is 0 when the environment sets
+ * PARATT_AMD_NO_SIDE_STREAM=1. */
+int ptt_hip_side_enabled(void);
+void ptt_hip_side_begin(void);
+void ptt_hip_side_end(int slot);
+void ptt_hip_side_wait(int slot);
+void ptt_hip_side_join(void);
 /* Grow-only device scratch slots owned by the library.  The returned pointer
  * is valid until the next call for the same slot with a larger size. */
 enum { PTT_WS_GRAM = 0, PTT_WS_SMALL, PTT_WS_TMP, PTT_WS_TMP2, PTT_WS_INT,
-       PTT_WS_COEF, PTT_WS_SKETCH, PTT_WS_SMALL2, PTT_WS_RSK, PTT_WS_NSLOTS };
+       PTT_WS_COEF, PTT_WS_SKETCH, PTT_WS_SMALL2, PTT_WS_RSK, PTT_WS_SIDE, PTT_WS_NSLOTS };
 void *ptt_hip_scratch(int slot, size_t bytes);
 /* Exchange the buffer of the TMP slot with a caller buffer: on return *buf
  * points at the old scratch (capacity *cap_bytes updated) and the slot owns

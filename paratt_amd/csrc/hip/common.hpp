@@ -12,6 +12,7 @@
 namespace ptt {
 
 extern hipStream_t g_stream;
+extern hipStream_t g_launch;  // g_stream, or the side stream between side_begin / side_end
 [[noreturn]] void fatal_hip(hipError_t e, const char *what, const char *file,
                             int line);
 
@@ -32,12 +33,12 @@ struct ProfScope {
     ~ProfScope();
 };
 
-// Launch on the library stream; with profiling on, HIP events bracket the
-// launch and its algorithmic flops / bytes are credited to `name`.
+// Launch on the current library stream; with profiling on, HIP events
+// bracket the launch and its algorithmic flops / bytes are credited to `name`.
 #define PTT_LAUNCH_FB(name, flops, bytes, kern, grid, block, shmem, ...) \
     do {                                                                \
         ptt::ProfScope ps_(name, flops, bytes);                         \
-        hipLaunchKernelGGL(kern, grid, block, shmem, ptt::g_stream,     \
+        hipLaunchKernelGGL(kern, grid, block, shmem, ptt::g_launch,     \
                            __VA_ARGS__);                                \
         HIPCHECK(hipGetLastError());                                    \
     } while (0)

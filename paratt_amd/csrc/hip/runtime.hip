@@ -10,6 +10,19 @@ namespace ptt {
 
 static bool g_init = false;
 hipStream_t g_stream = nullptr;
+// Second stream, for work that is off the critical path of the main stream
+// (both at default priority), and the stream that launches currently go to.
+hipStream_t g_side = nullptr;
+hipStream_t g_launch = nullptr;
+static hipEvent_t g_fork_ev = nullptr, g_join_ev = nullptr, g_side_ev[2] = {nullptr, nullptr};
+static bool g_side_used[2] = {false, false};
+static bool g_side_on = true;
+
+// Waits for both streams.
+static void sync_all() {
+    HIPCHECK(hipStreamSynchronize(g_stream));
+    if (g_side) HIPCHECK(hipStreamSynchronize(g_side));
+}
 // Host-visible mode (environment PARATT_AMD_HOST_VISIBLE=1, read once at
 // init): every library allocation is pinned host memory that the GPU reads
 // and writes through the same address, so host code may dereference
@@ -39,7 +52,7 @@ __global__ void k_copy_bytes(unsigned char *__restrict__ dst, const unsigned cha
 }
 
 static void host_mode_copy(void *dst, const void *src, size_t bytes) {
-    HIPCHECK(hipStreamSynchronize(g_stream));
+    sync_all();
     memcpy(dst, src, bytes);
 }
 
@@ -97,7 +110,7 @@ static hipEvent_t get_event() {
 
 static void resolve_pending() {
     if (g_pending.empty()) return;
-    HIPCHECK(hipStreamSynchronize(g_stream));
+    sync_all();  // events may sit on either stream
     for (auto &r : g_pending) {
         float ms = 0.f;
         HIPCHECK(hipEventElapsedTime(&ms, r.a, r.b));
@@ -117,12 +130,12 @@ ProfScope::ProfScope(const char *name, double f, double b)
       bytes(b) {
     if (!active) return;
     a = get_event();
-    HIPCHECK(hipEventRecord(a, g_stream));
+    HIPCHECK(hipEventRecord(a, g_launch));
 }
 ProfScope::~ProfScope() {
     if (!active) return;
     hipEvent_t b = get_event();
-    HIPCHECK(hipEventRecord(b, g_stream));
+    HIPCHECK(hipEventRecord(b, g_launch));
     g_pending.push_back({nm, a, b, flops, bytes});
     if (g_pending.size() > 4096) resolve_pending();
 }
@@ -141,8 +154,15 @@ void ptt_hip_init(void) {
         fprintf(stderr, "paratt_amd: no HIP device visible\n");
         exit(1);
     }
-    // Default-priority, non-blocking stream owned by the library.
+    // Default-priority, non-blocking streams owned by the library.
     HIPCHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
+    g_launch = g_stream;
+    HIPCHECK(hipEventCreateWithFlags(&g_fork_ev, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&g_join_ev, hipEventDisableTiming));
+    for (int i = 0; i < 2; i++) HIPCHECK(hipEventCreateWithFlags(&g_side_ev[i], hipEventDisableTiming));
+    const char *ns = getenv("PARATT_AMD_NO_SIDE_STREAM");
+    g_side_on = !(ns != nullptr && ns[0] != '\0' && ns[0] != '0');
     const char *hv = getenv("PARATT_AMD_HOST_VISIBLE");
     g_host_visible = hv != nullptr && hv[0] != '\0' && hv[0] != '0';
     HIPCHECK(hipHostMalloc((void **)&g_stage_d2h, STAGE_BYTES, hipHostMallocCoherent));
@@ -166,7 +186,7 @@ void *ptt_hip_malloc(size_t bytes) {
 
 void ptt_hip_free(void *p) {
     if (!p) return;
-    HIPCHECK(hipStreamSynchronize(g_stream));
+    sync_all();
     if (g_host_visible)
         HIPCHECK(hipHostFree(p));
     else
@@ -227,7 +247,36 @@ void ptt_hip_copy(void *dst, const void *src, size_t bytes) {
 
 void ptt_hip_sync(void) {
     ptt_hip_init();
-    HIPCHECK(hipStreamSynchronize(g_stream));
+    sync_all();
+}
+
+int ptt_hip_side_enabled(void) {
+    ptt_hip_init();
+    return g_side_on ? 1 : 0;
+}
+
+void ptt_hip_side_begin(void) {
+    ptt_hip_init();
+    HIPCHECK(hipEventRecord(g_fork_ev, g_stream));
+    HIPCHECK(hipStreamWaitEvent(g_side, g_fork_ev, 0));
+    g_launch = g_side;
+}
+
+void ptt_hip_side_end(int slot) {
+    HIPCHECK(hipEventRecord(g_side_ev[slot & 1], g_side));
+    g_side_used[slot & 1] = true;
+    g_launch = g_stream;
+}
+
+void ptt_hip_side_wait(int slot) {
+    ptt_hip_init();
+    if (g_side_used[slot & 1]) HIPCHECK(hipStreamWaitEvent(g_stream, g_side_ev[slot & 1], 0));
+}
+
+void ptt_hip_side_join(void) {
+    ptt_hip_init();
+    HIPCHECK(hipEventRecord(g_join_ev, g_side));
+    HIPCHECK(hipStreamWaitEvent(g_stream, g_join_ev, 0));
 }
 
 void *ptt_hip_scratch(int slot, size_t bytes) {

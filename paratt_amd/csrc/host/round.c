@@ -166,6 +166,53 @@ static void apply_left_t(attac_tt *x, int core, double *buf, int c, long len, co
     }
 }
 
+/* Q = A * T (A core `core`, len x c, c <= 128, in place) on the side stream:
+ * the left-to-right sweep's next bond does not read core `core`, so forming Q
+ * can run beside the next bond's one-workgroup Cholesky and its host round
+ * trip.  T is copied at once into one of two side buffers (the next bond's
+ * factorisation reuses the step workspace; a buffer is rewritten only after
+ * the side product that last read it has finished), and the product is
+ * queued: side_flush launches it once the next bond's Gram product is on
+ * the main stream (the side stream starts after it), or at the end of the
+ * sweep, which then joins the side stream. */
+static int g_side_parity = 0;
+static struct {
+    int pending, core, c, parity;
+    long len;
+    double alpha;
+    attac_tt *x;
+    double *tp;
+} g_side_q;
+
+static void side_flush(void) {
+    if (!g_side_q.pending) return;
+    g_side_q.pending = 0;
+    ptt_hip_side_begin();
+    ptt_hip_dgemm('N', 'N', (int)g_side_q.len, g_side_q.c, g_side_q.c, g_side_q.alpha,
+                  g_side_q.x->coreList[g_side_q.core], g_side_q.len, g_side_q.tp, g_side_q.c,
+                  0.0, g_side_q.x->coreList[g_side_q.core], g_side_q.len);
+    ptt_hip_side_end(g_side_q.parity);
+}
+
+static void apply_right_side(attac_tt *x, int core, long len, int c, const double *T,
+                             double alpha) {
+    side_flush();
+    double *ts = (double *)ptt_hip_scratch(PTT_WS_SIDE, sizeof(double) * 2 * 128 * 128);
+    const int p = g_side_parity;
+    g_side_parity ^= 1;
+    double *tp = ts + (size_t)p * 128 * 128;
+    ptt_hip_side_wait(p);
+    ptt_hip_d2d(tp, T, sizeof(double) * (size_t)c * c);
+    g_side_q.pending = 1;
+    g_side_q.x = x;
+    g_side_q.core = core;
+    g_side_q.len = len;
+    g_side_q.c = c;
+    g_side_q.alpha = alpha;
+    g_side_q.parity = p;
+    g_side_q.tp = tp;
+}
+
 /* Cholesky-QR of a core viewed as len x c (trans 'T': columns, LTR) or as
  * c x len (trans 'N': rows, RTL).  On return the core holds Q (alpha
  * applied) and R (c x c upper, A = Q*R resp. M = R^T*Q) is in *Rout.
@@ -175,6 +222,7 @@ static int cholqr_core(attac_tt *x, int core, double *buf, char trans, long len,
                        int *one_pass, const double **T1out) {
     double *A = buf ? buf : x->coreList[core];
     ptt_hip_gram(trans, c, len, A, trans == 'T' ? len : c, w->G);
+    side_flush(); /* the previous bond's Q runs beside this factorisation */
     chol_out o = run_chol(c, w->G, 0.0, w->R1, w->T1, w);
     if (!buf && trans == 'T' && core < 64) g_stats.kappa_ltr[core] = o.kappa;
     if (!buf && trans == 'N' && core < 64) g_stats.kappa_rtl[core] = o.kappa;
@@ -187,8 +235,14 @@ static int cholqr_core(attac_tt *x, int core, double *buf, char trans, long len,
             if (one_pass) *one_pass = 1;
             return 1;
         }
-        if (trans == 'T') apply_right(x, core, buf, len, c, w->T1, qalpha);
-        else apply_left_t(x, core, buf, c, len, w->T1, qalpha);
+        /* small products gain less from the overlap than the copy of T and
+         * the stream events cost */
+        if (trans == 'T' && !buf && c <= 128 && len * c >= (1L << 22) && ptt_hip_side_enabled())
+            apply_right_side(x, core, len, c, w->T1, qalpha);
+        else if (trans == 'T')
+            apply_right(x, core, buf, len, c, w->T1, qalpha);
+        else
+            apply_left_t(x, core, buf, c, len, w->T1, qalpha);
         *Rout = w->R1;
         return 1;
     }
@@ -281,6 +335,8 @@ static void ltr_sweep(attac_tt *x, const double *qscale, const double *rscale) {
             ptt_host_swap_core_with_tmp(x, i + 1, (long)c * wcols);
         }
     }
+    side_flush();
+    ptt_hip_side_join(); /* every Q formed on the side stream is in place */
 }
 
 /* Reference tail rule (pattac.c:468-481) on a descending list: the loop
