@@ -1014,17 +1014,18 @@ __global__ void k_svd_factors(int m, int rhomax, const int *__restrict__ drho,
 // preconditioner of Drmac and Veselic, SIAM J. Matrix Anal. Appl. 29, 2008,
 // applied statically).
 constexpr int SORT_MAXC = 1024;
-__global__ __launch_bounds__(256) void k_sym_sort_perm(int c, const double *__restrict__ G,
-                                                        double *__restrict__ Gp,
-                                                        int *__restrict__ perm) {
+__global__ __launch_bounds__(1024) void k_sym_sort_perm(int c, const double *__restrict__ G,
+                                                         double *__restrict__ Gp,
+                                                         int *__restrict__ perm) {
     __shared__ double sd[SORT_MAXC];
     __shared__ int sp[SORT_MAXC];
-    for (int j = threadIdx.x; j < c; j += blockDim.x) {
+    const int tid = threadIdx.x, hw = tid >> 5, hl = tid & 31, nhw = blockDim.x >> 5;
+    for (int j = tid; j < c; j += blockDim.x) {
         double v = G[j + (long)j * c];
         sd[j] = (v == v) ? v : -INFINITY;
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < c; j += blockDim.x) {
+    for (int j = tid; j < c; j += blockDim.x) {
         const double dj = sd[j];
         int pos = 0;
         for (int i = 0; i < c; i++) {
@@ -1035,9 +1036,10 @@ __global__ __launch_bounds__(256) void k_sym_sort_perm(int c, const double *__re
         perm[pos] = j;
     }
     __syncthreads();
-    for (long e = threadIdx.x; e < (long)c * c; e += blockDim.x) {
-        const int i = (int)(e % c), k = (int)(e / c);
-        Gp[e] = G[sp[i] + (long)sp[k] * c];
+    // one half-wave per column of Gp, 32 consecutive rows per pass
+    for (int k = hw; k < c; k += nhw) {
+        const double *gk = G + (long)sp[k] * c;
+        for (int i = hl; i < c; i += 32) Gp[i + (long)k * c] = gk[sp[i]];
     }
 }
 
@@ -1045,51 +1047,51 @@ __global__ __launch_bounds__(256) void k_sym_sort_perm(int c, const double *__re
 // sum to at most budget * max_j ||L_j||^2; the sum dropped is written to
 // *dropped.  Exactly-zero columns take no part in the Jacobi rotations, so
 // columns whose content is below the Gram's own rounding stop costing sweeps.
-__global__ __launch_bounds__(256) void k_drop_small_cols(int m, int k, double *__restrict__ L,
-                                                          long ld, double budget,
-                                                          double *__restrict__ dropped) {
+__global__ __launch_bounds__(1024) void k_drop_small_cols(int m, int k, double *__restrict__ L,
+                                                           long ld, double budget,
+                                                           double *__restrict__ dropped) {
     __shared__ double sn[SORT_MAXC];
     __shared__ int so[SORT_MAXC];
     __shared__ int sdrop[SORT_MAXC];
-    __shared__ double s_max;
-    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    const int tid = threadIdx.x, hw = tid >> 5, hl = tid & 31, nhw = blockDim.x >> 5;
+    // squared column norms, one half-wave per column (coalesced rows)
+    for (int j = hw; j < k; j += nhw) {
         double s = 0.0;
-        for (int i = 0; i < m; i++) {
+        for (int i = hl; i < m; i += 32) {
             const double v = L[i + j * ld];
             s += v * v;
         }
-        sn[j] = (s == s) ? s : INFINITY;
-        sdrop[j] = 0;
+        for (int o = 16; o > 0; o >>= 1) s += __shfl_xor(s, o, 32);
+        if (hl == 0) {
+            sn[j] = (s == s) ? s : INFINITY;
+            sdrop[j] = 0;
+        }
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    // ascending order (ties by index); the largest norm is the last one
+    for (int j = tid; j < k; j += blockDim.x) {
         const double sj = sn[j];
         int pos = 0;
         for (int i = 0; i < k; i++) pos += (sn[i] < sj) || (sn[i] == sj && i < j);
         so[pos] = j;
     }
-    if (threadIdx.x == 0) {
-        double mx = 0.0;
-        for (int j = 0; j < k; j++) mx = fmax(mx, sn[j]);
-        s_max = mx;
-    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const double lim = budget * s_max;
+    if (tid == 0) {
+        const double mx = fmax(sn[so[k - 1]], 0.0);
+        const double lim = budget * mx;
         double cum = 0.0;
         for (int q = 0; q < k; q++) {
             const double v = sn[so[q]];
-            if (!(cum + v <= lim) || v >= s_max) break;
+            if (!(cum + v <= lim) || v >= mx) break;
             cum += v;
             sdrop[so[q]] = 1;
         }
         *dropped = cum;
     }
     __syncthreads();
-    for (long e = threadIdx.x; e < (long)m * k; e += blockDim.x) {
-        const long j = e / m, i = e - j * m;
-        if (sdrop[j]) L[i + j * ld] = 0.0;
-    }
+    for (int j = hw; j < k; j += nhw)
+        if (sdrop[j])
+            for (int i = hl; i < m; i += 32) L[i + j * ld] = 0.0;
 }
 
 // dst[perm[i] + j*ldd] = src[i + j*lds], i < m, j < n.
@@ -1242,7 +1244,7 @@ void ptt_hip_sym_sort_perm(int c, const double *G, double *Gp, int *perm) {
         fprintf(stderr, "ptt_hip_sym_sort_perm: c = %d exceeds %d\n", c, SORT_MAXC);
         exit(1);
     }
-    PTT_LAUNCH("sym_sort_perm", k_sym_sort_perm, dim3(1), dim3(256), 0, c, G, Gp, perm);
+    PTT_LAUNCH("sym_sort_perm", k_sym_sort_perm, dim3(1), dim3(1024), 0, c, G, Gp, perm);
 }
 
 void ptt_hip_drop_small_cols(int m, int k, double *L, long ld, double budget, double *dropped) {
@@ -1252,7 +1254,7 @@ void ptt_hip_drop_small_cols(int m, int k, double *L, long ld, double budget, do
         fprintf(stderr, "ptt_hip_drop_small_cols: k = %d exceeds %d\n", k, SORT_MAXC);
         exit(1);
     }
-    PTT_LAUNCH("drop_small_cols", k_drop_small_cols, dim3(1), dim3(256), 0, m, k, L, ld, budget,
+    PTT_LAUNCH("drop_small_cols", k_drop_small_cols, dim3(1), dim3(1024), 0, m, k, L, ld, budget,
                dropped);
 }
 
