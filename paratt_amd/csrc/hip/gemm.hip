@@ -239,11 +239,11 @@ void gemm_impl(char ta, char tb, int M, int N, int K, double alpha, const double
         }
         return;
     }
-    // row Gram products (A A^T) of up to 128 rows use 64 x 64 tiles: four
-    // times the workgroups of one 128 x 128 tile per split, so every CU
-    // holds several and hides the streaming latency of its slab (measured
-    // faster for A A^T, slower for A^T A, whose slabs are column panels)
-    const bool small_tiles = force_split && M <= 128 && N <= 128 && (tb == 'T' || tb == 't');
+    // Gram products (A A^T and A^T A) of up to 128 columns use 64 x 64 tiles
+    // and about 512 workgroups over the K splits: several per CU, which
+    // hides the streaming latency of each slab (one 128 x 128 tile per CU
+    // leaves a quarter of its waves idle under the symmetry skip)
+    const bool small_tiles = force_split && M <= 128 && N <= 128;
     int BM = (M <= 64 || small_tiles) ? 64 : 128, BN = (N <= 64 || small_tiles) ? 64 : 128;
     // A 128 x 128 tile holds 360 registers per lane (one workgroup per CU);
     // 128 x 64 and 64 x 128 tiles fit two, which overlaps one tile's loads
@@ -257,8 +257,9 @@ void gemm_impl(char ta, char tb, int M, int N, int K, double alpha, const double
     int nsplit = 1;
     long tiles = (long)gm * gn;
     if ((force_split || tiles < 64) && K >= 1024) {
-        // one workgroup per CU (256) and >= 512 of K per split
-        long want = (256 + tiles - 1) / tiles;
+        // one workgroup per CU (256), two for the small-tile Grams, and
+        // >= 512 of K per split
+        long want = ((small_tiles ? 512 : 256) + tiles - 1) / tiles;
         long maxs = K / 512;
         nsplit = (int)(want < maxs ? want : maxs);
         if (nsplit < 1) nsplit = 1;
