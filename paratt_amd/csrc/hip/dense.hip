@@ -634,7 +634,8 @@ __device__ __forceinline__ void jacobi_cs(double al, double be, double ga, doubl
 // the pair rotated; cs / sn / unsettled as in the generic path.
 template <int GG, int RR>
 __device__ __forceinline__ bool jacobi_pair_fixed(double *ca, double *cb, bool valid, double tol,
-                                                  double &cs, double &sn, bool &unsettled) {
+                                                  double &cs, double &sn, bool &unsettled,
+                                                  bool &big) {
     double xa[RR], xb[RR];
     double al = 0.0, be = 0.0, ga = 0.0;
     if (valid) {
@@ -660,6 +661,7 @@ __device__ __forceinline__ bool jacobi_pair_fixed(double *ca, double *cb, bool v
     sn = 0.0;
     if (rot) jacobi_cs(al, be, ga, cs, sn);
     unsettled = rot && g2 > (4.0 * tol * tol) * ab;
+    big = rot && g2 > 1e-16 * ab;
     if (rot) {
 #pragma unroll
         for (int q = 0; q < RR; q++) {
@@ -764,12 +766,17 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
     // m-term dot product; below that the columns are orthogonal to working
     // accuracy (stricter thresholds never settle).
     const double tol = 0x1.0p-53 * (double)m;
+    __shared__ int s_big, s_bad;
     int sweep = 0;
     const int max_sweeps = 40;
     for (; sweep < max_sweeps && knz >= 2; sweep++) {
-        if (tid == 0) s_rot = 0;
+        if (tid == 0) {
+            s_rot = 0;
+            s_big = 0;
+            s_bad = 0;
+        }
         __syncthreads();
-        int rot_local = 0;
+        int rot_local = 0, big_local = 0;
         for (int t = 0; t < kp - 1; t++) {
             for (int p0 = 0; p0 < P; p0 += ngroups) {
                 int p = p0 + g;
@@ -831,13 +838,13 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
                     }
                 }
                 double cs = 1.0, sn = 0.0;
-                bool rot, unsettled;
+                bool rot, unsettled, big;
                 if (fix16) {
                     rot = jacobi_pair_fixed<16, SVD_RV>(ca + gl, cb + gl, valid, tol, cs, sn,
-                                                        unsettled);
+                                                        unsettled, big);
                 } else if (fix32) {
                     rot = jacobi_pair_fixed<32, SVD_RV / 2>(ca + gl, cb + gl, valid, tol, cs, sn,
-                                                            unsettled);
+                                                            unsettled, big);
                 } else {
                 group_allreduce3_q(al, be, ga, G);
                 const double nab = sqrt(al) * sqrt(be);
@@ -848,7 +855,9 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
                 // but only cosines above 2 tol keep the sweeps going;
                 // otherwise noise-triggered rotations never settle.
                 unsettled = rot && fabs(ga) > 2.0 * tol * nab;
+                big = rot && ga * ga > 1e-16 * (al * be);
                 }
+                if (big) big_local = 1;
                 if (LOGROT && p < P && gl == 0)
                     rotlog[((long)sweep * (kp - 1) + t) * P + p] = make_double2(cs, sn);
                 if (rot && !fix16 && !fix32) {
@@ -890,8 +899,71 @@ __global__ __launch_bounds__(SVD_NT) void k_svd_jacobi(int m, int k, const doubl
             __syncthreads();
         }
         if (rot_local) s_rot = 1;
+        if (big_local) s_big = 1;
         __syncthreads();
         if (!s_rot) break;
+        // When no pair of this sweep rotated by a cosine above 1e-8,
+        // quadratic convergence has usually left every pair orthogonal to
+        // working accuracy.  A direct test of every pair with the sweep's own
+        // criterion (unsettled: cos > 2 tol) then replaces the rotation-free
+        // sweep that would confirm it; if a pair fails, the sweeps go on.
+        // (The skipped sweep would only rotate pairs with tol < cos <= 2 tol.)
+        if (!s_big) {
+            double *nrm = s_sorted;  // free until the singular values are sorted
+            for (int j = tid; j < knz; j += nth) {
+                const double *cj = W + (long)j * ldw;
+                double a0 = 0.0, a1 = 0.0;
+                int i = 0;
+                for (; i + 1 < m; i += 2) {
+                    a0 += cj[i] * cj[i];
+                    a1 += cj[i + 1] * cj[i + 1];
+                }
+                if (i < m) a0 += cj[i] * cj[i];
+                nrm[j] = a0 + a1;
+            }
+            __syncthreads();
+            // 2 x 4 blocks of pair products over the upper triangle
+            const int nb2 = (knz + 1) >> 1, nb4 = (knz + 3) >> 2;
+            for (int t = tid; t < nb2 * nb4; t += nth) {
+                const int bi = t / nb4, bj = t - bi * nb4;
+                if (2 * bi > 4 * bj + 3) continue;
+                int oi[2], oj[4];
+#pragma unroll
+                for (int q = 0; q < 2; q++) oi[q] = min(2 * bi + q, knz - 1) * ldw;
+#pragma unroll
+                for (int q = 0; q < 4; q++) oj[q] = min(4 * bj + q, knz - 1) * ldw;
+                double acc[2][4];
+#pragma unroll
+                for (int a = 0; a < 2; a++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+                for (int r = 0; r < m; r++) {
+                    double x[2], y[4];
+#pragma unroll
+                    for (int q = 0; q < 2; q++) x[q] = W[oi[q] + r];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) y[q] = W[oj[q] + r];
+#pragma unroll
+                    for (int a = 0; a < 2; a++)
+#pragma unroll
+                        for (int b = 0; b < 4; b++) acc[a][b] += x[a] * y[b];
+                }
+                bool bad = false;
+#pragma unroll
+                for (int a = 0; a < 2; a++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const int ia = 2 * bi + a, jb = 4 * bj + b;
+                        if (ia < jb && jb < knz) {
+                            const double g2 = acc[a][b] * acc[a][b], ab = nrm[ia] * nrm[jb];
+                            bad = bad || (g2 > (4.0 * tol * tol) * ab);
+                        }
+                    }
+                if (bad) s_bad = 1;
+            }
+            __syncthreads();
+            if (!s_bad) break;
+        }
         __syncthreads();
     }
     // column norms
