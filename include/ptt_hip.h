@@ -157,7 +157,8 @@ void ptt_hip_svd_jacobi(int m, int k, double *A, long lda, double *sig,
                         double log_scale, double delta, int cap, int *iout);
 /* From U (m x k, ldu) and sig: W (rho x m, ld rho) = diag(1/sig) U_rho^T
  * scaled by wscale, and US (m x rho, ld m) = U_rho diag(sig).
- * rho is read from device *drho.  Either output may be NULL. */
+ * rho is read from device *drho, or is rhomax when drho is NULL.  Either
+ * output may be NULL. */
 void ptt_hip_svd_factors(int m, int rhomax, const int *drho, const double *U,
                          long ldu, const double *sig, double wscale, double *W,
                          double *US);

@@ -1067,7 +1067,7 @@ __global__ void k_svd_factors(int m, int rhomax, const int *__restrict__ drho,
                               const double *__restrict__ U, long ldu,
                               const double *__restrict__ sig, double wscale, double *W,
                               double *US) {
-    const int rho = *drho;
+    const int rho = drho ? *drho : rhomax;
     long total = (long)m * rhomax;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
          e += (long)gridDim.x * blockDim.x) {

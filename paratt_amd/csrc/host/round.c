@@ -442,10 +442,8 @@ static int rtl_step_fast(attac_tt *x, int i, const int *maxr, double delta,
     /* statistics: condition of the kept directions */
     if (i < 64) g_stats.kappa_rtl[i] = sig[rho0 - 1] > 0.0 ? s0 / sig[rho0 - 1] : 0.0;
     if (!(s0 > 0.0)) { free(hb); return -1; }
-    int h_rho0 = rho0;
-    ptt_hip_h2d(ws->iinfo + 8, &h_rho0, sizeof(int));
     double *tmp = (double *)ptt_hip_scratch(PTT_WS_TMP, sizeof(double) * (size_t)rho0 * w);
-    ptt_hip_svd_factors(rows, rho0, ws->iinfo + 8, ws->V, rows, ws->sig, nscale[i], ws->W, ws->US);
+    ptt_hip_svd_factors(rows, rho0, NULL, ws->V, rows, ws->sig, nscale[i], ws->W, ws->US);
     ptt_hip_dgemm('N', 'N', rho0, (int)w, rows, 1.0, ws->W, rho0, M, rows, 0.0, tmp, rho0);
     /* residual in place: M <- M - U Sigma (new core)/sqrt(n_i); tail^2 = ||M||^2 */
     ptt_hip_dgemm('N', 'N', rows, (int)w, rho0, -1.0 / nscale[i], ws->US, rows, tmp, rho0, 1.0, M,
