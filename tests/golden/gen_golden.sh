@@ -1,7 +1,7 @@
 #!/bin/bash
 # Regenerates tests/golden/golden_ref.json from the compiled reference.
-# Needs /root/reference (read in place) and the image's MPICH + MKL; run in
-# the build container, not on the GPU box.
+# Needs /root/reference (read in place) and the image's MPICH + MKL; run it
+# where those are present.  The tests only read the fixtures it writes.
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
 ROOT="$(cd "$HERE/../.." && pwd)"
