@@ -43,6 +43,7 @@ void ptt_hip_sync(void);  /* waits for the main and the side stream */
  * side_begin and side_end.  side_enabled is 0 when the environment sets
  * PARATT_AMD_NO_SIDE_STREAM=1. */
 int ptt_hip_side_enabled(void);
+void ptt_hip_side_enable(int on); /* switch the side stream on or off (waits first) */
 void ptt_hip_side_begin(void);
 void ptt_hip_side_end(int slot);
 void ptt_hip_side_wait(int slot);

@@ -255,6 +255,12 @@ int ptt_hip_side_enabled(void) {
     return g_side_on ? 1 : 0;
 }
 
+void ptt_hip_side_enable(int on) {
+    ptt_hip_init();
+    sync_all();
+    g_side_on = on != 0;
+}
+
 void ptt_hip_side_begin(void) {
     ptt_hip_init();
     HIPCHECK(hipEventRecord(g_fork_ev, g_stream));

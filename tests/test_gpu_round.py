@@ -2,6 +2,8 @@
 and against the CPU oracle on seeded inputs.  Bar: identical ranks, and
 TT-to-TT relative error (through the formal difference, no cancellation)
 <= 1e-12 plus the truncation eps where the cut is eps-bound."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -195,3 +197,38 @@ def test_round_headline_shape_matches_oracle(beta, d):
           "sensitivity to 1e-15 input noise %.3e)" % (d, beta, out.r[:4], rel, sens))
     assert rel < max(1e-12, 10 * sens)
     dz.free()
+
+
+def test_round_side_stream_bitwise_equal():
+    # The left-to-right sweep forms Q on a second stream beside the next
+    # bond's Cholesky when the product is large (here 131072 x 128 per core).
+    # The kernels and their arguments are the same with the second stream
+    # switched off, so the rounded train must agree bit for bit.
+    n, rk, d = 1024, 64, 6
+    sc = 1.0 / np.sqrt(n * rk)
+    ranks = O.uniform_ranks(d, rk)
+    x = O.gauss_tt([n] * d, ranks, 21, sc)
+    y = O.gauss_tt([n] * d, ranks, 22, sc)
+    z = O.axpby(1.0, x, 1.0, y)
+    cap = [1] + [64] * (d - 1) + [1]
+    tol = 1e-8 * O.norm(z)
+    L = P.lib()
+    L.ptt_hip_side_enabled.restype = C.c_int
+    L.ptt_hip_side_enable.argtypes = [C.c_int]
+    was = L.ptt_hip_side_enabled()
+    outs = []
+    try:
+        for on in (1, 0, 1):
+            L.ptt_hip_side_enable(on)
+            dz = P.DevTT.from_host(z)
+            P.truncate(dz, tol, cap)
+            outs.append(to_host(dz))
+            dz.free()
+    finally:
+        L.ptt_hip_side_enable(was)
+    for o in outs[1:]:
+        assert o.r == outs[0].r
+        for k in range(d):
+            a = outs[0].cores[k][: outs[0].core_size(k)]
+            b = o.cores[k][: o.core_size(k)]
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), k
